@@ -1,0 +1,141 @@
+// fe_internal.h — device buffers, launch plumbing and kernel launchers shared by the
+// C ABI (fe_api.cpp) and the kernels (fe_sort.hip, fe_sph.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fluid_engine.h"
+
+namespace fe {
+
+constexpr uint32_t kSentinel = 0xFFFFFFFFu;
+constexpr uint32_t kBlock = 256;          // threads per block of the per-particle kernels
+constexpr uint32_t kMaxSolveIters = 1024; // residual slots per solve
+
+// Per-kernel event profiling (enabled by fe_profile_enable).
+struct ProfileRecord {
+    std::string name;
+    hipEvent_t start, stop;
+};
+struct ProfileTotal {
+    std::string name;
+    uint64_t launches = 0;
+    double total_ms = 0.0;
+};
+
+// Scratch of the radix sort: per-tile digit counts and the digit totals of one pass.
+struct SortScratch {
+    uint32_t* counts = nullptr;   // 256 * max_tiles
+    uint32_t* totals = nullptr;   // 256
+    uint32_t max_tiles = 0;
+};
+
+struct Launch;
+
+}  // namespace fe
+
+// The context: SOA particle state in HBM, laid out as GpuPhysicsData
+// (src/entities/particle.rs:137-161), plus the internal neighbour-search arrays.
+struct fe_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    fe_params params{};
+    fe_options options{};
+    uint32_t n = 0;      // particles
+    uint32_t T = 0;      // hash-table entries = next_power_of_two(n)
+    uint32_t log2T = 0;
+
+    // GpuPhysicsData fields
+    float4* pos_a = nullptr;
+    float4* pos_b = nullptr;
+    float4* vel_a = nullptr;
+    float4* vel_b = nullptr;
+    float4* colors = nullptr;
+    float4* pacc = nullptr;
+    float* dens = nullptr;
+    float* fac = nullptr;
+    float* src = nullptr;
+    float* pres = nullptr;
+
+    // grid_entries as two arrays (hash, index) with a ping-pong partner for the sort;
+    // keys[0]/vals[0] always hold the current entries after a sort.
+    uint32_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    // grid_start (x) and one-past-the-end of each bucket (y); x = 0xFFFFFFFF when empty.
+    uint2* range = nullptr;
+    bool ends_dirty = false;   // y must be rebuilt from x and the keys (after uploads)
+
+    fe::SortScratch sort;
+
+    // stats_buffer and the convergence residuals (one slot per sweep, two solves)
+    uint32_t* stats = nullptr;
+    unsigned long long* resid = nullptr;   // 2 * kMaxSolveIters
+    uint32_t* host_stats = nullptr;         // pinned [3]
+    unsigned long long* host_resid = nullptr;  // pinned 2 * kMaxSolveIters
+    hipEvent_t stats_event = nullptr;
+    bool stats_pending = false;
+    bool have_prev_stats = false;
+    uint32_t prev_stats[3] = {0, 0, 0};
+
+    uint32_t last_iters[2] = {0, 0};
+    bool last_iters_pending[2] = {false, false};
+    uint64_t total_iters[2] = {0, 0};
+    uint64_t frames = 0;
+
+    // transfer staging for the interleaved entry layout
+    uint32_t* staging = nullptr;   // 2 * T
+
+    // timing
+    hipEvent_t timer_start = nullptr, timer_stop = nullptr;
+    bool profiling = false;
+    std::vector<fe::ProfileRecord> pending;
+    std::vector<fe::ProfileTotal> totals;
+};
+
+namespace fe {
+
+// RAII helper: brackets one kernel launch with profiling events when enabled.
+struct Launch {
+    fe_ctx* c;
+    ProfileRecord rec;
+    bool on;
+    Launch(fe_ctx* ctx, const char* name);
+    ~Launch();
+};
+
+// ---- launchers (fe_sort.hip) ----
+// Stable LSD radix sort of n (key, value) pairs on bits [0, key_bits) of the key.
+// The result is in keys[0]/vals[0] on return (buffers are swapped when needed).
+hipError_t radix_sort(fe_ctx* c, uint32_t n, uint32_t key_bits);
+// The bitonic network of bitonic_sort.comp over all T entries, in place in keys[0]/vals[0].
+hipError_t bitonic_sort(fe_ctx* c);
+hipError_t sort_scratch_alloc(SortScratch* s, uint32_t max_n);
+void sort_scratch_free(SortScratch* s);
+
+// ---- launchers (fe_sph.hip) ----
+hipError_t launch_hash(fe_ctx* c);
+hipError_t launch_offsets_reorder(fe_ctx* c);
+hipError_t launch_fill_ranges(fe_ctx* c);
+hipError_t launch_offsets_only(fe_ctx* c);
+hipError_t launch_rebuild_ends(fe_ctx* c);
+hipError_t launch_density_alpha(fe_ctx* c);
+hipError_t launch_viscosity(fe_ctx* c);
+hipError_t launch_density_source(fe_ctx* c);
+hipError_t launch_pressure_force(fe_ctx* c, const unsigned long long* prev, unsigned long long thr);
+hipError_t launch_pressure_update(fe_ctx* c, unsigned long long* out, const unsigned long long* prev,
+                                  unsigned long long thr);
+hipError_t launch_pressure_integration(fe_ctx* c);
+hipError_t launch_divergence_source(fe_ctx* c);
+hipError_t launch_divergence_integration(fe_ctx* c);
+hipError_t launch_stats(fe_ctx* c);
+hipError_t launch_fill_u64(fe_ctx* c, unsigned long long* p, uint32_t count);
+hipError_t launch_pack_entries(fe_ctx* c, uint32_t* out);
+hipError_t launch_unpack_entries(fe_ctx* c, const uint32_t* in);
+hipError_t launch_pack_start(fe_ctx* c, uint32_t* out);
+hipError_t launch_unpack_start(fe_ctx* c, const uint32_t* in);
+
+}  // namespace fe

@@ -1,0 +1,233 @@
+"""fluid_engine — MI355X-native DFSPH physics frame (HIP/CDNA4 kernels behind a C ABI).
+
+`Simulation` mirrors the reference's operator surface: one method per `ComputeStep`
+of `ComputePipelines` (src/renderer/pipelines/mod.rs:78-90), `step` for the physics part
+of `Renderer::step` (src/renderer/mod.rs:227-364), and field access to the buffers of
+`GpuPhysicsData` (src/entities/particle.rs:137-161). All work runs in libfluid_engine.so
+on the GPU; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_double, c_float, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+from . import scene
+from ._ffi import (EXPORTED_SYMBOLS, FE_SOLVER_CONVERGE, FE_SOLVER_FIXED, FE_SORT_BITONIC, FE_SORT_RADIX,
+                   FE_WARM_START_LEARNED, FE_WARM_START_REFERENCE, FIELDS, INT_FIELDS, LIB_PATH, FeOptions,
+                   FeParams, FluidEngineError, check, load_library)
+
+__all__ = [
+    "Simulation", "FeParams", "FeOptions", "FluidEngineError", "load_library", "scene", "frame_control",
+    "default_options", "FE_SORT_BITONIC", "FE_SORT_RADIX", "FE_SOLVER_FIXED", "FE_SOLVER_CONVERGE",
+    "FE_WARM_START_REFERENCE", "FE_WARM_START_LEARNED", "FIELDS", "EXPORTED_SYMBOLS", "LIB_PATH",
+]
+
+
+def default_options() -> FeOptions:
+    lib = load_library()
+    o = FeOptions()
+    check(lib.fe_default_options(byref(o)), "fe_default_options")
+    return o
+
+
+def frame_control(stats, n, params: FeParams, options: FeOptions):
+    """Host control of Renderer::step (src/renderer/mod.rs:231-267): returns
+    (dt, density_iterations, divergence_iterations) for the next frame."""
+    lib = load_library()
+    st = (c_uint32 * 3)(*[int(x) for x in stats])
+    dt = c_float()
+    di = c_uint32()
+    vi = c_uint32()
+    check(lib.fe_frame_control(st, n, byref(params), byref(options), byref(dt), byref(di), byref(vi)),
+          "fe_frame_control")
+    return dt.value, di.value, vi.value
+
+
+class Simulation:
+    """One fe_ctx: the particle state in HBM plus every physics stage."""
+
+    def __init__(self, params: FeParams, positions: np.ndarray, device: int = -1,
+                 options: FeOptions | None = None):
+        self._lib = load_library()
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        ctx = c_void_p()
+        check(self._lib.fe_create(byref(params), pos.ctypes.data, pos.shape[0], device, byref(ctx)), "fe_create")
+        self._ctx = ctx
+        n = c_uint32()
+        t = c_uint32()
+        check(self._lib.fe_num_particles(ctx, byref(n), byref(t)), "fe_num_particles")
+        self.n = n.value
+        self.table_size = t.value
+        if options is not None:
+            self.set_options(options)
+
+    # -- lifetime -------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            check(self._lib.fe_destroy(self._ctx), "fe_destroy")
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- parameters / options -------------------------------------------------------
+    @property
+    def params(self) -> FeParams:
+        p = FeParams()
+        check(self._lib.fe_get_params(self._ctx, byref(p)), "fe_get_params")
+        return p
+
+    def set_params(self, params: FeParams):
+        check(self._lib.fe_set_params(self._ctx, byref(params)), "fe_set_params")
+
+    @property
+    def options(self) -> FeOptions:
+        o = FeOptions()
+        check(self._lib.fe_get_options(self._ctx, byref(o)), "fe_get_options")
+        return o
+
+    def set_options(self, options: FeOptions):
+        check(self._lib.fe_set_options(self._ctx, byref(options)), "fe_set_options")
+
+    def configure(self, **kw):
+        o = self.options
+        for k, v in kw.items():
+            setattr(o, k, v)
+        self.set_options(o)
+
+    # -- fields -------------------------------------------------------------------------
+    def _shape(self, name):
+        fid, per, per_table = FIELDS[name]
+        count = self.table_size if per_table else self.n
+        return fid, (count, per) if per > 1 else (count,)
+
+    def download(self, name: str) -> np.ndarray:
+        fid, shape = self._shape(name)
+        out = np.empty(shape, dtype=np.uint32 if name in INT_FIELDS else np.float32)
+        check(self._lib.fe_download(self._ctx, fid, out.ctypes.data, out.nbytes), f"fe_download({name})")
+        return out
+
+    def upload(self, name: str, data) -> None:
+        fid, shape = self._shape(name)
+        arr = np.ascontiguousarray(data, dtype=np.uint32 if name in INT_FIELDS else np.float32).reshape(shape)
+        check(self._lib.fe_upload(self._ctx, fid, arr.ctypes.data, arr.nbytes), f"fe_upload({name})")
+
+    def device_ptr(self, name: str) -> int:
+        p = c_void_p()
+        check(self._lib.fe_device_ptr(self._ctx, FIELDS[name][0], byref(p)), "fe_device_ptr")
+        return p.value
+
+    # -- ComputeSteps -------------------------------------------------------------------
+    def neighbor_search(self):
+        check(self._lib.fe_neighbor_search(self._ctx), "fe_neighbor_search")
+
+    def sort(self):
+        check(self._lib.fe_sort(self._ctx), "fe_sort")
+
+    def density_alpha(self):
+        check(self._lib.fe_density_alpha(self._ctx), "fe_density_alpha")
+
+    def viscosity(self):
+        check(self._lib.fe_viscosity(self._ctx), "fe_viscosity")
+
+    def density_source(self):
+        check(self._lib.fe_density_source(self._ctx), "fe_density_source")
+
+    def pressure_force(self):
+        check(self._lib.fe_pressure_force(self._ctx), "fe_pressure_force")
+
+    def pressure_update(self):
+        check(self._lib.fe_pressure_update(self._ctx), "fe_pressure_update")
+
+    def pressure_integration(self):
+        check(self._lib.fe_pressure_integration(self._ctx), "fe_pressure_integration")
+
+    def divergence_source(self):
+        check(self._lib.fe_divergence_source(self._ctx), "fe_divergence_source")
+
+    def divergence_integration(self):
+        check(self._lib.fe_divergence_integration(self._ctx), "fe_divergence_integration")
+
+    def stats(self):
+        check(self._lib.fe_stats(self._ctx), "fe_stats")
+
+    def solve_density(self) -> int:
+        it = c_uint32()
+        check(self._lib.fe_solve_density(self._ctx, byref(it)), "fe_solve_density")
+        return it.value
+
+    def solve_divergence(self) -> int:
+        it = c_uint32()
+        check(self._lib.fe_solve_divergence(self._ctx, byref(it)), "fe_solve_divergence")
+        return it.value
+
+    def step(self, max_dt: float | None = None):
+        dt = self.params.dt if max_dt is None else max_dt
+        check(self._lib.fe_step(self._ctx, c_float(dt)), "fe_step")
+
+    def run(self, frames: int, max_dt: float | None = None):
+        dt = self.params.dt if max_dt is None else max_dt
+        check(self._lib.fe_run(self._ctx, c_float(dt), frames), "fe_run")
+
+    # -- readback / timing ----------------------------------------------------------------
+    def read_stats(self) -> np.ndarray:
+        out = (c_uint32 * 3)()
+        check(self._lib.fe_read_stats(self._ctx, out), "fe_read_stats")
+        return np.array(out[:], dtype=np.uint32)
+
+    def last_iterations(self):
+        d = c_uint32()
+        v = c_uint32()
+        check(self._lib.fe_last_iterations(self._ctx, byref(d), byref(v)), "fe_last_iterations")
+        return d.value, v.value
+
+    def iteration_totals(self):
+        d = c_uint64()
+        v = c_uint64()
+        f = c_uint64()
+        check(self._lib.fe_iteration_totals(self._ctx, byref(d), byref(v), byref(f)), "fe_iteration_totals")
+        return d.value, v.value, f.value
+
+    def reset_iteration_totals(self):
+        check(self._lib.fe_reset_iteration_totals(self._ctx), "fe_reset_iteration_totals")
+
+    def sync(self):
+        check(self._lib.fe_sync(self._ctx), "fe_sync")
+
+    def timer_start(self):
+        check(self._lib.fe_timer_start(self._ctx), "fe_timer_start")
+
+    def timer_stop(self) -> float:
+        ms = c_float()
+        check(self._lib.fe_timer_stop(self._ctx, byref(ms)), "fe_timer_stop")
+        return ms.value
+
+    def profile(self, enable: bool = True):
+        check(self._lib.fe_profile_enable(self._ctx, 1 if enable else 0), "fe_profile_enable")
+
+    def profile_reset(self):
+        check(self._lib.fe_profile_reset(self._ctx), "fe_profile_reset")
+
+    def profile_results(self) -> dict:
+        cnt = c_uint32()
+        check(self._lib.fe_profile_count(self._ctx, byref(cnt)), "fe_profile_count")
+        res = {}
+        buf = ctypes.create_string_buffer(128)
+        for i in range(cnt.value):
+            launches = c_uint64()
+            total = c_double()
+            check(self._lib.fe_profile_entry(self._ctx, i, buf, 128, byref(launches), byref(total)),
+                  "fe_profile_entry")
+            res[buf.value.decode()] = (launches.value, total.value)
+        return res

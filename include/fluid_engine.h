@@ -1,0 +1,218 @@
+/*
+ * fluid_engine.h — C ABI of the MI355X-native DFSPH physics frame.
+ *
+ * Drop-in boundary for the physics half of Nikita-Lysiuk/Fluid-Engine:
+ *   - the `ComputeStep` operator trait (src/renderer/pipelines/mod.rs:52-76) and the
+ *     `ComputePipelines` aggregate (:78-124) become one `fe_ctx` with one enqueue call
+ *     per reference ComputeStep;
+ *   - `GpuPhysicsData` (src/entities/particle.rs:137-308) becomes the device buffers the
+ *     context owns, reachable through fe_upload / fe_download;
+ *   - `SimulationParams` (src/entities/particle.rs:310-331) is `fe_params`, byte-identical
+ *     (112 bytes, std140, 16-byte aligned), so a Rust caller can pass `&SimulationParams`;
+ *   - `Renderer::step` (src/renderer/mod.rs:227-364) physics part is `fe_step`.
+ *
+ * Conventions: every call returns an fe_status (0 = FE_OK); no exception crosses the ABI;
+ * the message of the last failure on the calling thread is in fe_last_error().
+ * All work is enqueued on the context's own HIP stream (default priority) in call order,
+ * which replaces vulkano's automatic barriers; calls on one context are not thread-safe,
+ * separate contexts may be driven from separate threads. fe_sync() waits for the stream.
+ * Host arrays belong to the caller; uploads and downloads copy.
+ */
+#ifndef FLUID_ENGINE_H
+#define FLUID_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------------------- */
+typedef int32_t fe_status;
+#define FE_OK 0
+#define FE_ERR_INVALID_ARGUMENT 1
+#define FE_ERR_OUT_OF_MEMORY 2
+#define FE_ERR_DEVICE 3
+#define FE_ERR_UNSUPPORTED 4
+#define FE_ERR_SIZE_MISMATCH 5
+
+/* ---- parameter block ---------------------------------------------------------------------
+ * Byte layout of `SimulationParams` (src/entities/particle.rs:310-331) and of the std140
+ * uniform block in shaders/include/common.glsl:11-28. Offsets: 0 particle_radius,
+ * 4 particle_mass, 8 smoothing_radius, 12 target_density, 16 viscosity, 20 relax_factor,
+ * 24 dt, 28 density_solver_iterations, 32 divergence_solver_iterations, 36 pad[3],
+ * 48 gravity, 64 box_min, 80 box_max, 96 grid_res. */
+typedef struct fe_params {
+    float particle_radius;
+    float particle_mass;
+    float smoothing_radius;
+    float target_density;
+    float viscosity;
+    float relax_factor;
+    float dt;
+    uint32_t density_solver_iterations;
+    uint32_t divergence_solver_iterations;
+    float _pad[3];
+    float gravity[4];
+    float box_min[4];
+    float box_max[4];
+    int32_t grid_res[4];
+} fe_params;
+
+/* Sort switch: `SortAlgorithm` (src/renderer/pipelines/sorter.rs:19-23). */
+#define FE_SORT_BITONIC 0u
+#define FE_SORT_RADIX 1u
+
+/* Pressure-solver termination. FIXED runs density_solver_iterations /
+ * divergence_solver_iterations Jacobi sweeps as the reference does
+ * (src/renderer/mod.rs:301-306, :327-332). CONVERGE runs sweeps until the
+ * compression-only mean density error is <= density_tol (resp. divergence_tol),
+ * at least 1 and at most max_iterations sweeps; this mode has no reference counterpart. */
+#define FE_SOLVER_FIXED 0u
+#define FE_SOLVER_CONVERGE 1u
+
+/* Initial pressure of the two Jacobi solves. REFERENCE: zero before the density solve
+ * (shaders/compute/density_source_term.comp:73) and the density solve's last pressure
+ * before the divergence solve (pressures is never reset, src/renderer/mod.rs:323-332).
+ * LEARNED: the per-particle MLP warm start (fe_warm_start_* below). */
+#define FE_WARM_START_REFERENCE 0u
+#define FE_WARM_START_LEARNED 1u
+
+/* Run options. The first five mirror the runtime flags of `AppUI`
+ * (src/renderer/ui.rs:12-49): sort_algorithm, use_cfl, use_solver_error_threshold,
+ * density_error_pct, divergence_error_pct, whose host logic is in Renderer::step
+ * (src/renderer/mod.rs:231-267). */
+typedef struct fe_options {
+    uint32_t sort_algorithm;             /* FE_SORT_RADIX (app default, ui.rs:36) */
+    uint32_t use_cfl;                    /* 0 */
+    uint32_t use_solver_error_threshold; /* 0: early-exit to 2/1 sweeps, mod.rs:251-265 */
+    float density_error_pct;             /* 0.1 */
+    float divergence_error_pct;          /* 0.5 */
+    uint32_t solver_mode;                /* FE_SOLVER_FIXED */
+    float density_tol;                   /* 1e-3 (relative) for FE_SOLVER_CONVERGE */
+    float divergence_tol;                /* 1e-2 (relative) for FE_SOLVER_CONVERGE */
+    uint32_t max_iterations;             /* 100 */
+    uint32_t warm_start;                 /* FE_WARM_START_REFERENCE */
+    uint32_t reserved[6];
+} fe_options;
+
+/* Device buffers of `GpuPhysicsData` (src/entities/particle.rs:137-161).
+ * Element size per particle: vec4 fields 16 B (w as the reference writes it),
+ * scalar fields 4 B; GRID_ENTRIES is T x {uint32 hash, uint32 index} (8 B) and
+ * GRID_START is T x uint32, with T = next_power_of_two(n) (particle.rs:255). */
+#define FE_FIELD_POSITION_A 0
+#define FE_FIELD_POSITION_B 1
+#define FE_FIELD_VELOCITY_A 2
+#define FE_FIELD_VELOCITY_B 3
+#define FE_FIELD_COLORS 4
+#define FE_FIELD_DENSITIES 5
+#define FE_FIELD_FACTORS 6
+#define FE_FIELD_SOURCE_TERMS 7
+#define FE_FIELD_PRESSURES 8
+#define FE_FIELD_PRESSURE_ACCELERATIONS 9
+#define FE_FIELD_GRID_ENTRIES 10
+#define FE_FIELD_GRID_START 11
+#define FE_FIELD_COUNT 12
+
+typedef struct fe_ctx fe_ctx;
+
+/* Library version string and the last error message of the calling thread. */
+const char* fe_version(void);
+const char* fe_last_error(void);
+
+/* Replaces GpuPhysicsData::new (particle.rs:164-299) + ComputePipelines::new
+ * (pipelines/mod.rs:93-123) + the prepare() calls (src/renderer/mod.rs:137-192):
+ * xyz is n x 3 floats; position_a = (xyz, 1), velocities and pressures zero.
+ * Uses HIP device `device` (the current device if negative). */
+fe_status fe_create(const fe_params* params, const float* xyz, uint32_t n, int device,
+                    fe_ctx** out);
+fe_status fe_destroy(fe_ctx* ctx);
+
+/* GpuSceneResources::sync_with_scene (src/renderer/resources.rs:85-92). */
+fe_status fe_set_params(fe_ctx* ctx, const fe_params* params);
+fe_status fe_get_params(const fe_ctx* ctx, fe_params* out);
+fe_status fe_set_options(fe_ctx* ctx, const fe_options* options);
+fe_status fe_get_options(const fe_ctx* ctx, fe_options* out);
+/* Fills `out` with the defaults listed in fe_options. */
+fe_status fe_default_options(fe_options* out);
+
+fe_status fe_num_particles(const fe_ctx* ctx, uint32_t* n, uint32_t* table_size);
+
+/* Copy a whole field between host and device (bytes must equal the field size). */
+fe_status fe_upload(fe_ctx* ctx, int field, const void* src, size_t bytes);
+fe_status fe_download(fe_ctx* ctx, int field, void* dst, size_t bytes);
+/* Device address of a field for zero-copy interop (valid for the context's lifetime).
+ * GRID_ENTRIES and GRID_START are stored in a different internal layout and are not
+ * exposed this way (FE_ERR_UNSUPPORTED). */
+fe_status fe_device_ptr(fe_ctx* ctx, int field, void** out);
+
+/* ---- one enqueue per reference ComputeStep::execute --------------------------------------- */
+/* NeighborSearch::execute (neighbor_search.rs:176-231): hash, sort, offsets, reorder. */
+fe_status fe_neighbor_search(fe_ctx* ctx);
+/* The sort stage alone on GRID_ENTRIES (RadixSorter::execute sorter.rs:325-455 or
+ * GpuSorter::execute sorter.rs:73-115, by options.sort_algorithm). */
+fe_status fe_sort(fe_ctx* ctx);
+fe_status fe_density_alpha(fe_ctx* ctx);            /* density_and_alpha.comp */
+fe_status fe_viscosity(fe_ctx* ctx);                /* viscosity.comp */
+fe_status fe_density_source(fe_ctx* ctx);           /* density_source_term.comp */
+fe_status fe_pressure_force(fe_ctx* ctx);           /* pressure_force.comp */
+fe_status fe_pressure_update(fe_ctx* ctx);          /* pressure_update.comp */
+fe_status fe_pressure_integration(fe_ctx* ctx);     /* pressure_integration.comp */
+fe_status fe_divergence_source(fe_ctx* ctx);        /* divergence_source_term.comp */
+fe_status fe_divergence_integration(fe_ctx* ctx);   /* divergence_integration.comp */
+fe_status fe_stats(fe_ctx* ctx);                    /* fill 0 + stats.comp */
+
+/* Jacobi solves: density (after fe_density_source) and divergence (after
+ * fe_divergence_source), FIXED or CONVERGE per the options. `iterations` (may be NULL)
+ * receives the number of sweeps that did work; in CONVERGE mode this waits for the
+ * stream. */
+fe_status fe_solve_density(fe_ctx* ctx, uint32_t* iterations);
+fe_status fe_solve_divergence(fe_ctx* ctx, uint32_t* iterations);
+
+/* The physics frame of Renderer::step (src/renderer/mod.rs:227-364): frame-start neighbour
+ * search and density+alpha, substeps while t < max_dt, then stats. With max_dt == dt it is
+ * exactly one substep. Host flags (CFL, early exit) read the previous frame's stats. */
+fe_status fe_step(fe_ctx* ctx, float max_dt);
+/* Run `count` frames back to back without host synchronisation (FE_SOLVER_FIXED and no
+ * host flags), or with the per-frame control when those are on. */
+fe_status fe_run(fe_ctx* ctx, float max_dt, uint32_t count);
+
+/* stats_buffer (particle.rs:270-281): [max_speed_bits, density_error_fp, divergence_error_fp]. */
+fe_status fe_read_stats(fe_ctx* ctx, uint32_t out[3]);
+/* Iteration counts of the last frame's density / divergence solves. */
+fe_status fe_last_iterations(fe_ctx* ctx, uint32_t* density, uint32_t* divergence);
+/* Cumulative sweep counts since creation or the last reset. */
+fe_status fe_iteration_totals(fe_ctx* ctx, uint64_t* density, uint64_t* divergence,
+                              uint64_t* frames);
+fe_status fe_reset_iteration_totals(fe_ctx* ctx);
+
+fe_status fe_sync(fe_ctx* ctx);
+
+/* ---- timing on the context's stream --------------------------------------------------------
+ * fe_timer_start/stop bracket enqueued work with HIP events on the context's stream;
+ * fe_timer_stop waits and returns the elapsed milliseconds. */
+fe_status fe_timer_start(fe_ctx* ctx);
+fe_status fe_timer_stop(fe_ctx* ctx, float* ms);
+/* Per-kernel profiling: when enabled, every kernel launch of the context is bracketed by
+ * events and its duration accumulated under its kernel name. */
+fe_status fe_profile_enable(fe_ctx* ctx, int enable);
+/* Number of profiled kernel names. */
+fe_status fe_profile_count(fe_ctx* ctx, uint32_t* count);
+/* Entry i: name (NUL-terminated, truncated to name_cap), launch count, total ms. Waits. */
+fe_status fe_profile_entry(fe_ctx* ctx, uint32_t i, char* name, size_t name_cap,
+                           uint64_t* launches, double* total_ms);
+fe_status fe_profile_reset(fe_ctx* ctx);
+
+/* ---- host frame control (src/renderer/mod.rs:231-267) -------------------------------------
+ * Pure host function: from the previous frame's stats decide the next dt (CFL) and the two
+ * sweep counts (early exit). Exposed for tests. */
+fe_status fe_frame_control(const uint32_t stats[3], uint32_t n, const fe_params* params,
+                           const fe_options* options, float* dt_out,
+                           uint32_t* density_iters, uint32_t* divergence_iters);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUID_ENGINE_H */

@@ -1,0 +1,837 @@
+// fe_oracle.cpp — CPU restatement of the reference DFSPH frame (the oracle).
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the HIP path; never the product path.
+//
+// Each function cites the reference file:line it restates. FP32 arithmetic follows the
+// GLSL expression order exactly (built with -ffp-contract=off, IEEE division and sqrt);
+// FP64 contexts run the same expressions in double. OpenMP parallelises the per-particle
+// loops; every particle's sum is accumulated serially in the reference visit order, so
+// results do not depend on the thread count.
+#include "fe_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr uint32_t kSentinel = 0xFFFFFFFFu;
+
+template <class R>
+struct V3 {
+    R x, y, z;
+};
+template <class R>
+inline V3<R> sub(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R>
+inline V3<R> add(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R>
+inline V3<R> scale(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R>
+inline V3<R> scale_r(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+// GLSL dot(a, b), evaluated as (x*x' + y*y') + z*z'.
+template <class R>
+inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// Literals of common.glsl / the shaders: float literals in FP32, their decimal value in FP64.
+template <class R> struct Lit;
+template <> struct Lit<float> {
+    static constexpr float pi = 3.1415926535897932384626433832795f;
+    static constexpr float eps6 = 1e-6f;
+    static constexpr float eps20 = 1e-20f;
+};
+template <> struct Lit<double> {
+    static constexpr double pi = 3.1415926535897932384626433832795;
+    static constexpr double eps6 = 1e-6;
+    static constexpr double eps20 = 1e-20;
+};
+
+// common.glsl:30-35 get_cell_hash. uint(ivec3) is a two's-complement reinterpretation.
+inline uint32_t cell_hash(int32_t cx, int32_t cy, int32_t cz, uint32_t T) {
+    return ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u ^ (uint32_t)cz * 83492791u) % T;
+}
+
+// spatial_hash.comp:29 / density_and_alpha.comp:27: ivec3(floor(pos / h)), always in FP32.
+// The vec3-by-scalar division is evaluated as pos * (1 / h) with the correctly rounded
+// reciprocal: that is the form the reference GPU executed, fixed by the bucket order of
+// scripts/rest_density.csv (all 4800 rows reproduce in place only with it; the IEEE
+// quotient moves 240 lattice particles that sit exactly on a cell face).
+inline float inv_h_of(float h) { return 1.0f / h; }
+inline void cell_of(float px, float py, float pz, float h, int32_t c[3]) {
+    const float ih = inv_h_of(h);
+    c[0] = (int32_t)std::floor(px * ih);
+    c[1] = (int32_t)std::floor(py * ih);
+    c[2] = (int32_t)std::floor(pz * ih);
+}
+
+// common.glsl:38-54 kernel_w (cubic spline, support h).
+template <class R>
+R kernel_w(R r, R h) {
+    R q = r / h;
+    if (q >= R(1)) return R(0);
+    R h3 = h * h * h;
+    R k = R(8) / (Lit<R>::pi * h3);
+    if (q <= R(0.5)) {
+        R q2 = q * q;
+        R q3 = q2 * q;
+        return k * (R(6) * q3 - R(6) * q2 + R(1));
+    }
+    R omq = R(1) - q;
+    R omq3 = omq * omq * omq;
+    return k * R(2) * omq3;
+}
+
+// common.glsl:56-73 kernel_grad.
+template <class R>
+V3<R> kernel_grad(V3<R> rv, R r, R h) {
+    R q = r / h;
+    if (q >= R(1) || r < Lit<R>::eps6) return {R(0), R(0), R(0)};
+    R h3 = h * h * h;
+    R k = R(8) / (Lit<R>::pi * h3);
+    R gf;
+    if (q <= R(0.5)) {
+        gf = k * (R(18) * q * q - R(12) * q);
+    } else {
+        R omq = R(1) - q;
+        gf = k * R(-6) * omq * omq;
+    }
+    return scale(gf / (h * r), rv);
+}
+
+template <class R>
+struct State {
+    fe_params p{};
+    fe_options o{};
+    uint32_t n = 0, T = 0;
+    std::vector<R> pos_a, pos_b, vel_a, vel_b, colors, pacc;  // 4*n
+    std::vector<R> dens, fac, src, pres;                      // n
+    std::vector<uint32_t> entries;                            // 2*T, {hash, index}
+    std::vector<uint32_t> grid_start;                         // T
+    uint32_t stats[3] = {0, 0, 0};
+    uint32_t last_iters[2] = {0, 0};
+    bool have_stats = false;
+
+    void alloc(uint32_t n_) {
+        n = n_;
+        T = 1;
+        while (T < n) T <<= 1;
+        for (auto* v : {&pos_a, &pos_b, &vel_a, &vel_b, &colors, &pacc}) v->assign(4 * (size_t)n, R(0));
+        for (auto* v : {&dens, &fac, &src, &pres}) v->assign(n, R(0));
+        entries.assign(2 * (size_t)T, 0);
+        grid_start.assign(T, 0);
+    }
+    V3<R> v3(const std::vector<R>& a, uint32_t i) const { return {a[4 * i], a[4 * i + 1], a[4 * i + 2]}; }
+    void put4(std::vector<R>& a, uint32_t i, V3<R> v, R w) {
+        a[4 * i] = v.x; a[4 * i + 1] = v.y; a[4 * i + 2] = v.z; a[4 * i + 3] = w;
+    }
+    R P(float x) const { return R(x); }
+};
+
+// Visit the 27 cells around particle i (z outer, y, x inner) and every sorted entry j of
+// each bucket (density_and_alpha.comp:31-45 and the same skeleton in the other gathers).
+// f(j) is called in the reference visit order, including repeated buckets when two
+// neighbour cells hash to the same slot.
+template <class R, class F>
+inline void for_each_candidate(const State<R>& s, uint32_t i, F&& f) {
+    const float h = s.p.smoothing_radius;
+    int32_t c[3];
+    cell_of((float)s.pos_b[4 * i], (float)s.pos_b[4 * i + 1], (float)s.pos_b[4 * i + 2], h, c);
+    for (int z = -1; z <= 1; z++)
+        for (int y = -1; y <= 1; y++)
+            for (int x = -1; x <= 1; x++) {
+                uint32_t key = cell_hash(c[0] + x, c[1] + y, c[2] + z, s.T);
+                uint32_t start = s.grid_start[key];
+                if (start == kSentinel) continue;
+                for (uint32_t j = start; j < s.n; j++) {
+                    if (s.entries[2 * (size_t)j] != key) break;
+                    f(j);
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------------------
+// Neighbour search: spatial_hash.comp:23-36, sort, grid_offsets.comp:20-32, reorder.comp:37-45
+// driven as in NeighborSearch::execute (neighbor_search.rs:176-231).
+
+template <class R>
+void spatial_hash(State<R>& s) {
+    const float h = s.p.smoothing_radius;
+    for (uint32_t i = 0; i < s.T; i++) {
+        if (i < s.n) {
+            int32_t c[3];
+            cell_of((float)s.pos_a[4 * i], (float)s.pos_a[4 * i + 1], (float)s.pos_a[4 * i + 2], h, c);
+            s.entries[2 * (size_t)i] = cell_hash(c[0], c[1], c[2], s.T);
+            s.entries[2 * (size_t)i + 1] = i;
+        } else {
+            s.entries[2 * (size_t)i] = kSentinel;
+            s.entries[2 * (size_t)i + 1] = kSentinel;
+        }
+    }
+}
+
+// radix_count.comp:26-47, radix_scan.comp:18-51, radix_reorder.comp:35-97 with the
+// RadixSorter::execute driver (sorter.rs:325-455): 4 passes of 8 bits, WG = 256,
+// EPI = 8 (chunk 2048), counts stored column-major [digit * num_wg + wg], ping-pong
+// between the entry buffer and a temporary; the result lands back in the entry buffer.
+void radix_sort_literal(uint32_t* pairs, uint32_t n) {
+    const uint32_t WG = 256, EPI = 8, CHUNK = WG * EPI;
+    const uint32_t num_wg = (n + CHUNK - 1) / CHUNK;
+    std::vector<uint32_t> tmp(2 * (size_t)n);
+    std::vector<uint32_t> counts(256 * (size_t)num_wg);
+    uint32_t* bufs[2] = {pairs, tmp.data()};
+    for (uint32_t pass = 0; pass < 4; pass++) {
+        const uint32_t shift = 8 * pass;
+        const uint32_t* in = bufs[pass & 1];
+        uint32_t* out = bufs[(pass & 1) ^ 1];
+        // count: one WG per chunk, every invocation counts EPI consecutive elements.
+        for (uint32_t wg = 0; wg < num_wg; wg++) {
+            uint32_t local[256] = {0};
+            for (uint32_t lid = 0; lid < WG; lid++) {
+                uint32_t base = wg * WG * EPI + lid * EPI;
+                for (uint32_t e = 0; e < EPI; e++) {
+                    uint32_t gi = base + e;
+                    if (gi >= n) break;
+                    local[(in[2 * (size_t)gi] >> shift) & 0xFFu]++;
+                }
+            }
+            for (uint32_t d = 0; d < 256; d++) counts[(size_t)d * num_wg + wg] = local[d];
+        }
+        // scan: each digit's stripe serially (exclusive), then an inclusive prefix over the
+        // 256 stripe totals added back as block offsets.
+        uint32_t sums[256];
+        for (uint32_t d = 0; d < 256; d++) {
+            uint32_t running = 0;
+            for (uint32_t k = d * num_wg; k < (d + 1) * num_wg; k++) {
+                uint32_t v = counts[k];
+                counts[k] = running;
+                running += v;
+            }
+            sums[d] = running;
+        }
+        for (uint32_t stride = 1; stride < 256; stride *= 2) {
+            uint32_t prev[256];
+            std::memcpy(prev, sums, sizeof(sums));
+            for (uint32_t d = stride; d < 256; d++) sums[d] += prev[d - stride];
+        }
+        for (uint32_t d = 0; d < 256; d++) {
+            uint32_t block_offset = d == 0 ? 0 : sums[d - 1];
+            for (uint32_t k = d * num_wg; k < (d + 1) * num_wg; k++) counts[k] += block_offset;
+        }
+        // reorder: invocation `lid` of WG `wg` walks the whole chunk in order and writes the
+        // elements of digit `lid` to consecutive slots from its global offset (stable).
+        for (uint32_t wg = 0; wg < num_wg; wg++) {
+            uint32_t base = wg * CHUNK;
+            uint32_t pos[256];
+            for (uint32_t d = 0; d < 256; d++) pos[d] = counts[(size_t)d * num_wg + wg];
+            for (uint32_t e = 0; e < CHUNK; e++) {
+                uint32_t gi = base + e;
+                if (gi >= n) continue;
+                uint32_t d = (in[2 * (size_t)gi] >> shift) & 0xFFu;
+                uint32_t o = pos[d]++;
+                out[2 * (size_t)o] = in[2 * (size_t)gi];
+                out[2 * (size_t)o + 1] = in[2 * (size_t)gi + 1];
+            }
+        }
+    }
+}
+
+// bitonic_sort.comp:19-43 driven by GpuSorter::execute (sorter.rs:73-115): for every
+// block height h = 2..n and step = h/2..1, invocation i < n/2 compares
+// l = (i/step)*step*2 + i%step with l+step, ascending iff (l & h) == 0, and swaps the whole
+// entry when out of order. Unstable but deterministic.
+void bitonic_sort_literal(uint32_t* pairs, uint32_t n) {
+    for (uint32_t h = 2; h <= n && h != 0; h *= 2) {
+        for (uint32_t step = h / 2; step > 0; step /= 2) {
+            for (uint32_t i = 0; i < n / 2; i++) {
+                uint32_t l = (i / step) * step * 2 + (i % step);
+                uint32_t r = l + step;
+                uint32_t kl = pairs[2 * (size_t)l], kr = pairs[2 * (size_t)r];
+                bool asc = (l & h) == 0;
+                if ((asc && kl > kr) || (!asc && kl < kr)) {
+                    std::swap(pairs[2 * (size_t)l], pairs[2 * (size_t)r]);
+                    std::swap(pairs[2 * (size_t)l + 1], pairs[2 * (size_t)r + 1]);
+                }
+            }
+        }
+        if (h == 0x80000000u) break;
+    }
+}
+
+template <class R>
+void sort_entries(State<R>& s) {
+    if (s.o.sort_algorithm == FE_SORT_BITONIC)
+        bitonic_sort_literal(s.entries.data(), s.T);
+    else
+        radix_sort_literal(s.entries.data(), s.T);
+}
+
+// neighbor_search.rs:207-208 fill_buffer(grid_start, 0xFFFFFFFF) + grid_offsets.comp:20-32.
+template <class R>
+void grid_offsets(State<R>& s) {
+    std::fill(s.grid_start.begin(), s.grid_start.end(), kSentinel);
+    for (uint32_t i = 0; i < s.T; i++) {
+        uint32_t key = s.entries[2 * (size_t)i];
+        if (key == kSentinel) continue;
+        uint32_t prev = i == 0 ? kSentinel : s.entries[2 * (size_t)(i - 1)];
+        if (key != prev) s.grid_start[key] = i;
+    }
+}
+
+// reorder.comp:37-45: pos_b[i] = pos_a[e[i].index], vel_b[i] = vel_a[e[i].index], i < N.
+template <class R>
+void reorder(State<R>& s) {
+    for (uint32_t i = 0; i < s.n; i++) {
+        uint32_t src = s.entries[2 * (size_t)i + 1];
+        for (int k = 0; k < 4; k++) {
+            s.pos_b[4 * (size_t)i + k] = s.pos_a[4 * (size_t)src + k];
+            s.vel_b[4 * (size_t)i + k] = s.vel_a[4 * (size_t)src + k];
+        }
+    }
+}
+
+template <class R>
+void neighbor_search(State<R>& s) {
+    spatial_hash(s);
+    sort_entries(s);
+    grid_offsets(s);
+    reorder(s);
+}
+
+// ---------------------------------------------------------------------------------------
+// Solver kernels.
+
+// density_and_alpha.comp:14-72.
+template <class R>
+void density_alpha(State<R>& s) {
+    const R h = s.P(s.p.smoothing_radius), mass = s.P(s.p.particle_mass);
+    const R hh = h * h;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s.n; ii++) {
+        uint32_t i = (uint32_t)ii;
+        V3<R> pi = s.v3(s.pos_b, i);
+        R density = R(0), sum_grad_sq = R(0);
+        V3<R> grad_sum{R(0), R(0), R(0)};
+        for_each_candidate(s, i, [&](uint32_t j) {
+            V3<R> rv = sub(pi, s.v3(s.pos_b, j));
+            R r2 = dot(rv, rv);
+            if (r2 > hh) return;
+            R r = std::sqrt(r2);
+            R w = kernel_w(r, h);
+            density += mass * w;
+            if (r > Lit<R>::eps6) {
+                V3<R> g = kernel_grad(rv, r, h);
+                V3<R> mg = scale(mass, g);
+                grad_sum = add(grad_sum, mg);
+                sum_grad_sq += dot(mg, mg);
+            }
+        });
+        s.dens[i] = density;
+        R alpha = sum_grad_sq + dot(grad_sum, grad_sum);
+        s.fac[i] = alpha > Lit<R>::eps6 ? alpha : R(0);
+    }
+}
+
+// viscosity.comp:17-66 (reads velocity_b, writes velocity_a with w = 0; gravity fused).
+template <class R>
+void viscosity(State<R>& s) {
+    const R h = s.P(s.p.smoothing_radius), mass = s.P(s.p.particle_mass);
+    const R hh = h * h, nu = s.P(s.p.viscosity), dt = s.P(s.p.dt);
+    const V3<R> g{s.P(s.p.gravity[0]), s.P(s.p.gravity[1]), s.P(s.p.gravity[2])};
+    std::vector<R> out(4 * (size_t)s.n);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s.n; ii++) {
+        uint32_t i = (uint32_t)ii;
+        V3<R> pi = s.v3(s.pos_b, i);
+        V3<R> sum{R(0), R(0), R(0)};
+        for_each_candidate(s, i, [&](uint32_t j) {
+            V3<R> rv = sub(pi, s.v3(s.pos_b, j));
+            R r2 = dot(rv, rv);
+            if (r2 > hh) return;
+            R r = std::sqrt(r2);
+            R w = kernel_w(r, h);
+            if (r > Lit<R>::eps6) {
+                V3<R> vd = sub(s.v3(s.vel_b, j), s.v3(s.vel_b, i));
+                R m_rho = mass / s.dens[j];
+                sum = add(sum, scale_r(scale(m_rho, vd), w));
+            }
+        });
+        V3<R> v = s.v3(s.vel_b, i);
+        V3<R> visco = add(v, scale(nu, sum));
+        V3<R> fin = add(visco, scale_r(g, dt));
+        out[4 * (size_t)i] = fin.x; out[4 * (size_t)i + 1] = fin.y;
+        out[4 * (size_t)i + 2] = fin.z; out[4 * (size_t)i + 3] = R(0);
+    }
+    s.vel_a.swap(out);
+}
+
+// density_source_term.comp:16-74 (velocity_a; pressures zero-initialised).
+template <class R>
+void density_source(State<R>& s) {
+    const R h = s.P(s.p.smoothing_radius), mass = s.P(s.p.particle_mass);
+    const R hh = h * h, dt = s.P(s.p.dt), rho0 = s.P(s.p.target_density);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s.n; ii++) {
+        uint32_t i = (uint32_t)ii;
+        V3<R> pi = s.v3(s.pos_b, i), vi = s.v3(s.vel_a, i);
+        R rho_i = s.dens[i];
+        R div = R(0);
+        for_each_candidate(s, i, [&](uint32_t j) {
+            V3<R> rv = sub(pi, s.v3(s.pos_b, j));
+            R r2 = dot(rv, rv);
+            if (r2 > hh) return;
+            R r = std::sqrt(r2);
+            if (r > Lit<R>::eps6) {
+                V3<R> g = kernel_grad(rv, r, h);
+                V3<R> vd = sub(vi, s.v3(s.vel_a, j));
+                div += mass * dot(vd, g);
+            }
+        });
+        R source = R(0);
+        if (dt > Lit<R>::eps6) source = ((rho0 - rho_i) / dt) - div;
+        s.src[i] = source;
+        s.pres[i] = R(0);
+    }
+}
+
+// pressure_force.comp:15-78.
+template <class R>
+void pressure_force(State<R>& s) {
+    const R h = s.P(s.p.smoothing_radius), mass = s.P(s.p.particle_mass);
+    const R hh = h * h;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s.n; ii++) {
+        uint32_t i = (uint32_t)ii;
+        V3<R> pi = s.v3(s.pos_b, i);
+        R rho_i = s.dens[i], p_i = s.pres[i];
+        R p_rho_i = R(0);
+        if (rho_i > Lit<R>::eps6) p_rho_i = p_i / (rho_i * rho_i);
+        V3<R> acc{R(0), R(0), R(0)};
+        for_each_candidate(s, i, [&](uint32_t j) {
+            V3<R> rv = sub(pi, s.v3(s.pos_b, j));
+            R r2 = dot(rv, rv);
+            if (r2 > hh) return;
+            R r = std::sqrt(r2);
+            if (r > Lit<R>::eps6) {
+                R rho_j = s.dens[j], p_j = s.pres[j];
+                if (rho_j > Lit<R>::eps6) {
+                    V3<R> g = kernel_grad(rv, r, h);
+                    R p_rho_j = p_j / (rho_j * rho_j);
+                    R term = p_rho_i + p_rho_j;
+                    acc = add(acc, scale(mass * term, g));
+                }
+            }
+        });
+        s.put4(s.pacc, i, V3<R>{-acc.x, -acc.y, -acc.z}, R(0));
+    }
+}
+
+// Convergence residual quantisation (no reference counterpart; documented in the header).
+inline uint64_t residual_q(float s_minus_ap, float dt, float rho0) {
+    float neg = -s_minus_ap;
+    if (!(neg > 0.0f)) return 0;
+    float e = (neg * dt) / rho0;
+    if (!(e < 256.0f)) e = 256.0f;
+    return (uint64_t)(e * 4294967296.0f);
+}
+
+// pressure_update.comp:18-84 (in place; neighbours' pressures are not read).
+template <class R>
+uint64_t pressure_update(State<R>& s) {
+    const R h = s.P(s.p.smoothing_radius), mass = s.P(s.p.particle_mass);
+    const R hh = h * h, dt = s.P(s.p.dt), omega = s.P(s.p.relax_factor);
+    uint64_t resid = 0;
+#pragma omp parallel for schedule(static) reduction(+ : resid)
+    for (int64_t ii = 0; ii < (int64_t)s.n; ii++) {
+        uint32_t i = (uint32_t)ii;
+        V3<R> pi = s.v3(s.pos_b, i), ai = s.v3(s.pacc, i);
+        R alpha_i = s.fac[i], rho_i = s.dens[i], src_i = s.src[i], p_i = s.pres[i];
+        R sum_ap = R(0);
+        for_each_candidate(s, i, [&](uint32_t j) {
+            V3<R> rv = sub(pi, s.v3(s.pos_b, j));
+            R r2 = dot(rv, rv);
+            if (r2 > hh) return;
+            R r = std::sqrt(r2);
+            if (r > Lit<R>::eps6) {
+                V3<R> g = kernel_grad(rv, r, h);
+                V3<R> ad = sub(ai, s.v3(s.pacc, j));
+                sum_ap += mass * dot(ad, g);
+            }
+        });
+        R ap = dt * sum_ap;
+        R err = src_i - ap;
+        resid += residual_q((float)err, (float)dt, s.p.target_density);
+        if (alpha_i > Lit<R>::eps6 && rho_i > Lit<R>::eps6) {
+            R a_ii = -(dt / (rho_i * rho_i)) * alpha_i;
+            if (std::fabs(a_ii) > Lit<R>::eps20) {
+                R corr = (omega * err) / a_ii;
+                R np = p_i + corr;
+                s.pres[i] = np > R(0) ? np : R(0);
+            }
+        }
+    }
+    return resid;
+}
+
+// pressure_integration.comp:16-77 (reads position_b, writes position_a; collision order
+// y, x, z with restitution 0.5, friction 0.95, eps 0.001).
+template <class R>
+void pressure_integration(State<R>& s) {
+    const R dt = s.P(s.p.dt), rad = s.P(s.p.particle_radius);
+    const R mn[3] = {s.P(s.p.box_min[0]), s.P(s.p.box_min[1]), s.P(s.p.box_min[2])};
+    const R mx[3] = {s.P(s.p.box_max[0]), s.P(s.p.box_max[1]), s.P(s.p.box_max[2])};
+    const R rest = R(0.5), fric = R(0.95), eps = R(0.001);
+    const int order[3] = {1, 0, 2};
+    for (uint32_t i = 0; i < s.n; i++) {
+        R pos[3], vel[3], ap[3];
+        for (int k = 0; k < 3; k++) {
+            pos[k] = s.pos_b[4 * (size_t)i + k];
+            vel[k] = s.vel_a[4 * (size_t)i + k];
+            ap[k] = s.pacc[4 * (size_t)i + k];
+        }
+        R nv[3], np[3];
+        for (int k = 0; k < 3; k++) nv[k] = vel[k] + ap[k] * dt;
+        for (int k = 0; k < 3; k++) np[k] = pos[k] + nv[k] * dt;
+        for (int a : order) {
+            int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+            if (np[a] < mn[a] + rad) {
+                np[a] = mn[a] + rad + eps;
+                nv[a] *= -rest;
+                nv[t1] *= fric;
+                nv[t2] *= fric;
+            } else if (np[a] > mx[a] - rad) {
+                np[a] = mx[a] - rad - eps;
+                nv[a] *= -rest;
+                nv[t1] *= fric;
+                nv[t2] *= fric;
+            }
+        }
+        s.put4(s.pos_a, i, V3<R>{np[0], np[1], np[2]}, R(1));
+        s.put4(s.vel_a, i, V3<R>{nv[0], nv[1], nv[2]}, R(0));
+    }
+}
+
+// divergence_source_term.comp:14-62 (velocity_a indexed by the current sort order).
+template <class R>
+void divergence_source(State<R>& s) {
+    const R h = s.P(s.p.smoothing_radius), mass = s.P(s.p.particle_mass);
+    const R hh = h * h;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s.n; ii++) {
+        uint32_t i = (uint32_t)ii;
+        V3<R> pi = s.v3(s.pos_b, i), vi = s.v3(s.vel_a, i);
+        R div = R(0);
+        for_each_candidate(s, i, [&](uint32_t j) {
+            V3<R> rv = sub(pi, s.v3(s.pos_b, j));
+            R r2 = dot(rv, rv);
+            if (r2 > hh) return;
+            R r = std::sqrt(r2);
+            if (r > Lit<R>::eps6) {
+                V3<R> g = kernel_grad(rv, r, h);
+                V3<R> vd = sub(vi, s.v3(s.vel_a, j));
+                div += mass * dot(vd, g);
+            }
+        });
+        s.src[i] = -div;
+    }
+}
+
+// GLSL mix(x, y, a) = x * (1 - a) + y * a.
+template <class R>
+inline R mixf(R x, R y, R a) { return x * (R(1) - a) + y * a; }
+
+// divergence_integration.comp:14-43.
+template <class R>
+void divergence_integration(State<R>& s) {
+    const R dt = s.P(s.p.dt);
+    const R deep[3] = {R(0.05), R(0.1), R(0.5)};
+    const R water[3] = {R(0.2), R(0.5), R(1.0)};
+    const R foam[3] = {R(1), R(1), R(1)};
+    for (uint32_t i = 0; i < s.n; i++) {
+        V3<R> v = s.v3(s.vel_a, i), a = s.v3(s.pacc, i);
+        V3<R> nv = add(v, scale_r(a, dt));
+        R speed = std::sqrt(dot(nv, nv));
+        R t = speed / R(4);
+        t = std::min(std::max(t, R(0)), R(1));
+        R col[3];
+        for (int k = 0; k < 3; k++)
+            col[k] = t < R(0.5) ? mixf(deep[k], water[k], t * R(2))
+                                : mixf(water[k], foam[k], (t - R(0.5)) * R(2));
+        s.put4(s.vel_a, i, nv, R(0));
+        s.put4(s.colors, i, V3<R>{col[0], col[1], col[2]}, R(1));
+    }
+}
+
+// uint(x) for x >= 0 with the conversion saturating at the uint range (NaN -> 0).
+inline uint32_t to_u32_sat(float x) {
+    if (!(x > 0.0f)) return 0;
+    if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+// stats_pipeline.rs:56-66 fill 0 + stats.comp:18-30.
+template <class R>
+void stats(State<R>& s) {
+    uint32_t st[3] = {0, 0, 0};
+    const float rho0 = s.p.target_density;
+    for (uint32_t i = 0; i < s.n; i++) {
+        float vx = (float)s.vel_a[4 * (size_t)i], vy = (float)s.vel_a[4 * (size_t)i + 1],
+              vz = (float)s.vel_a[4 * (size_t)i + 2];
+        float speed = std::sqrt(vx * vx + vy * vy + vz * vz);
+        uint32_t bits;
+        std::memcpy(&bits, &speed, 4);
+        st[0] = std::max(st[0], bits);
+        float de = std::fabs((float)s.dens[i] - rho0);
+        st[1] += std::min(to_u32_sat(de * 1.0f), 10000u);
+        float ve = std::fabs((float)s.src[i]);
+        st[2] += std::min(to_u32_sat(ve * 10.0f), 10000u);
+    }
+    std::memcpy(s.stats, st, sizeof(st));
+    s.have_stats = true;
+}
+
+// Jacobi loop: src/renderer/mod.rs:301-306 (density) and :327-332 (divergence); CONVERGE
+// mode stops on the quantised compression residual of the sweep.
+template <class R>
+uint32_t solve(State<R>& s, uint32_t fixed_iters, float tol) {
+    if (s.o.solver_mode != FE_SOLVER_CONVERGE) {
+        for (uint32_t k = 0; k < fixed_iters; k++) {
+            pressure_force(s);
+            pressure_update(s);
+        }
+        return fixed_iters;
+    }
+    const uint64_t thr = (uint64_t)((double)tol * (double)s.n * 4294967296.0);
+    uint32_t maxit = s.o.max_iterations ? s.o.max_iterations : 1;
+    for (uint32_t k = 0; k < maxit; k++) {
+        pressure_force(s);
+        uint64_t q = pressure_update(s);
+        if (q <= thr) return k + 1;
+    }
+    return maxit;
+}
+
+// Renderer::step host control (src/renderer/mod.rs:231-267) — CFL dt and early exit.
+template <class R>
+void frame_control(State<R>& s, uint32_t* dit, uint32_t* vit) {
+    *dit = s.p.density_solver_iterations;
+    *vit = s.p.divergence_solver_iterations;
+    // stats_buffer starts as [0, 0, 0] (particle.rs:270-281) and is read every frame.
+    float n = (float)s.n;
+    float max_speed;
+    std::memcpy(&max_speed, &s.stats[0], 4);
+    float avg_de = (float)s.stats[1] / (1.0f * n);
+    float avg_ve = (float)s.stats[2] / (10.0f * n);
+    if (s.o.use_cfl && max_speed > 0.01f) {
+        float h = s.p.smoothing_radius;
+        float cfl = std::min(std::max(0.4f * h / max_speed, 0.001f), 0.05f);
+        s.p.dt = std::min(cfl, s.p.dt * 1.1f);
+    }
+    float rho0 = s.p.target_density;
+    float dthr = s.o.density_error_pct / 100.0f * rho0;
+    float vthr = s.o.divergence_error_pct / 100.0f * rho0;
+    if (s.o.use_solver_error_threshold && avg_de < dthr) *dit = 2;
+    if (s.o.use_solver_error_threshold && avg_ve < vthr) *vit = 1;
+}
+
+// Renderer::step physics frame (src/renderer/mod.rs:279-342).
+template <class R>
+void step(State<R>& s, float max_dt) {
+    uint32_t dit, vit;
+    frame_control(s, &dit, &vit);
+    neighbor_search(s);
+    density_alpha(s);
+    float t = 0.0f;
+    while (t < max_dt) {
+        viscosity(s);
+        density_source(s);
+        s.last_iters[0] = solve(s, dit, s.o.density_tol);
+        pressure_integration(s);
+        t += s.p.dt;
+        neighbor_search(s);
+        density_alpha(s);
+        divergence_source(s);
+        s.last_iters[1] = solve(s, vit, s.o.divergence_tol);
+        divergence_integration(s);
+    }
+    stats(s);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// C interface.
+
+struct oracle_ctx {
+    int fp64;
+    State<float> f;
+    State<double> d;
+};
+
+#define DISPATCH(c, call) ((c)->fp64 ? (call((c)->d)) : (call((c)->f)))
+
+template <class R>
+static std::vector<R>* fp_field(State<R>& s, int field, size_t* per) {
+    *per = 4;
+    switch (field) {
+        case FE_FIELD_POSITION_A: return &s.pos_a;
+        case FE_FIELD_POSITION_B: return &s.pos_b;
+        case FE_FIELD_VELOCITY_A: return &s.vel_a;
+        case FE_FIELD_VELOCITY_B: return &s.vel_b;
+        case FE_FIELD_COLORS: return &s.colors;
+        case FE_FIELD_PRESSURE_ACCELERATIONS: return &s.pacc;
+        default: break;
+    }
+    *per = 1;
+    switch (field) {
+        case FE_FIELD_DENSITIES: return &s.dens;
+        case FE_FIELD_FACTORS: return &s.fac;
+        case FE_FIELD_SOURCE_TERMS: return &s.src;
+        case FE_FIELD_PRESSURES: return &s.pres;
+        default: return nullptr;
+    }
+}
+
+template <class R>
+static int upload_impl(State<R>& s, int field, const void* src, size_t bytes) {
+    if (field == FE_FIELD_GRID_ENTRIES) {
+        if (bytes != 8 * (size_t)s.T) return 1;
+        std::memcpy(s.entries.data(), src, bytes);
+        return 0;
+    }
+    if (field == FE_FIELD_GRID_START) {
+        if (bytes != 4 * (size_t)s.T) return 1;
+        std::memcpy(s.grid_start.data(), src, bytes);
+        return 0;
+    }
+    size_t per;
+    std::vector<R>* v = fp_field(s, field, &per);
+    if (!v || bytes != 4 * per * (size_t)s.n) return 1;
+    const float* f = (const float*)src;
+    for (size_t k = 0; k < per * (size_t)s.n; k++) (*v)[k] = (R)f[k];
+    return 0;
+}
+
+template <class R>
+static int download_impl(State<R>& s, int field, void* dst, size_t bytes, int as_double) {
+    if (field == FE_FIELD_GRID_ENTRIES) {
+        if (bytes != 8 * (size_t)s.T) return 1;
+        std::memcpy(dst, s.entries.data(), bytes);
+        return 0;
+    }
+    if (field == FE_FIELD_GRID_START) {
+        if (bytes != 4 * (size_t)s.T) return 1;
+        std::memcpy(dst, s.grid_start.data(), bytes);
+        return 0;
+    }
+    size_t per;
+    std::vector<R>* v = fp_field(s, field, &per);
+    if (!v) return 1;
+    size_t cnt = per * (size_t)s.n;
+    if (as_double) {
+        if (bytes != 8 * cnt) return 1;
+        double* o = (double*)dst;
+        for (size_t k = 0; k < cnt; k++) o[k] = (double)(*v)[k];
+    } else {
+        if (bytes != 4 * cnt) return 1;
+        float* o = (float*)dst;
+        for (size_t k = 0; k < cnt; k++) o[k] = (float)(*v)[k];
+    }
+    return 0;
+}
+
+extern "C" {
+
+oracle_ctx* oracle_create(const fe_params* params, const float* xyz, uint32_t n, int fp64) {
+    oracle_ctx* c = new oracle_ctx();
+    c->fp64 = fp64 ? 1 : 0;
+    fe_options o;
+    std::memset(&o, 0, sizeof(o));
+    o.sort_algorithm = FE_SORT_RADIX;
+    o.density_error_pct = 0.1f;
+    o.divergence_error_pct = 0.5f;
+    o.density_tol = 1e-3f;
+    o.divergence_tol = 1e-2f;
+    o.max_iterations = 100;
+    auto init = [&](auto& s) {
+        s.p = *params;
+        s.o = o;
+        s.alloc(n);
+        for (uint32_t i = 0; i < n; i++) {
+            for (int k = 0; k < 3; k++) s.pos_a[4 * (size_t)i + k] = xyz[3 * (size_t)i + k];
+            s.pos_a[4 * (size_t)i + 3] = 1;
+        }
+    };
+    if (c->fp64) init(c->d); else init(c->f);
+    return c;
+}
+
+void oracle_destroy(oracle_ctx* c) { delete c; }
+void oracle_set_params(oracle_ctx* c, const fe_params* p) { c->f.p = *p; c->d.p = *p; }
+void oracle_set_options(oracle_ctx* c, const fe_options* o) { c->f.o = *o; c->d.o = *o; }
+void oracle_set_threads(int t) {
+#ifdef _OPENMP
+    omp_set_num_threads(t > 0 ? t : 1);
+#else
+    (void)t;
+#endif
+}
+
+int oracle_upload(oracle_ctx* c, int field, const void* src, size_t bytes) {
+    return c->fp64 ? upload_impl(c->d, field, src, bytes) : upload_impl(c->f, field, src, bytes);
+}
+int oracle_download(oracle_ctx* c, int field, void* dst, size_t bytes, int as_double) {
+    return c->fp64 ? download_impl(c->d, field, dst, bytes, as_double)
+                   : download_impl(c->f, field, dst, bytes, as_double);
+}
+
+void oracle_neighbor_search(oracle_ctx* c) { DISPATCH(c, neighbor_search); }
+void oracle_sort(oracle_ctx* c) { DISPATCH(c, sort_entries); }
+void oracle_density_alpha(oracle_ctx* c) { DISPATCH(c, density_alpha); }
+void oracle_viscosity(oracle_ctx* c) { DISPATCH(c, viscosity); }
+void oracle_density_source(oracle_ctx* c) { DISPATCH(c, density_source); }
+void oracle_pressure_force(oracle_ctx* c) { DISPATCH(c, pressure_force); }
+uint64_t oracle_pressure_update(oracle_ctx* c) {
+    return c->fp64 ? pressure_update(c->d) : pressure_update(c->f);
+}
+void oracle_pressure_integration(oracle_ctx* c) { DISPATCH(c, pressure_integration); }
+void oracle_divergence_source(oracle_ctx* c) { DISPATCH(c, divergence_source); }
+void oracle_divergence_integration(oracle_ctx* c) { DISPATCH(c, divergence_integration); }
+void oracle_stats(oracle_ctx* c) { DISPATCH(c, stats); }
+void oracle_read_stats(oracle_ctx* c, uint32_t out[3]) {
+    std::memcpy(out, c->fp64 ? c->d.stats : c->f.stats, 12);
+}
+
+uint32_t oracle_solve_density(oracle_ctx* c) {
+    if (c->fp64) return c->d.last_iters[0] = solve(c->d, c->d.p.density_solver_iterations, c->d.o.density_tol);
+    return c->f.last_iters[0] = solve(c->f, c->f.p.density_solver_iterations, c->f.o.density_tol);
+}
+uint32_t oracle_solve_divergence(oracle_ctx* c) {
+    if (c->fp64) return c->d.last_iters[1] = solve(c->d, c->d.p.divergence_solver_iterations, c->d.o.divergence_tol);
+    return c->f.last_iters[1] = solve(c->f, c->f.p.divergence_solver_iterations, c->f.o.divergence_tol);
+}
+void oracle_step(oracle_ctx* c, float max_dt) {
+    if (c->fp64) step(c->d, max_dt); else step(c->f, max_dt);
+}
+void oracle_last_iterations(oracle_ctx* c, uint32_t* density, uint32_t* divergence) {
+    const uint32_t* it = c->fp64 ? c->d.last_iters : c->f.last_iters;
+    *density = it[0];
+    *divergence = it[1];
+}
+
+uint32_t oracle_cell_hash(int32_t cx, int32_t cy, int32_t cz, uint32_t T) { return cell_hash(cx, cy, cz, T); }
+float oracle_kernel_w(float r, float h) { return kernel_w<float>(r, h); }
+double oracle_kernel_w64(double r, double h) { return kernel_w<double>(r, h); }
+void oracle_kernel_grad(const float rv[3], float r, float h, float out[3]) {
+    V3<float> g = kernel_grad<float>(V3<float>{rv[0], rv[1], rv[2]}, r, h);
+    out[0] = g.x; out[1] = g.y; out[2] = g.z;
+}
+void oracle_radix_sort_pairs(uint32_t* pairs, uint32_t n) { radix_sort_literal(pairs, n); }
+void oracle_bitonic_sort_pairs(uint32_t* pairs, uint32_t n) { bitonic_sort_literal(pairs, n); }
+uint64_t oracle_residual_q(float s_minus_ap, float dt, float rho0) { return residual_q(s_minus_ap, dt, rho0); }
+
+}  // extern "C"
