@@ -772,6 +772,7 @@ oracle_ctx* oracle_create(const fe_params* params, const float* xyz, uint32_t n,
 
 void oracle_destroy(oracle_ctx* c) { delete c; }
 void oracle_set_params(oracle_ctx* c, const fe_params* p) { c->f.p = *p; c->d.p = *p; }
+void oracle_get_params(oracle_ctx* c, fe_params* out) { *out = c->fp64 ? c->d.p : c->f.p; }
 void oracle_set_options(oracle_ctx* c, const fe_options* o) { c->f.o = *o; c->d.o = *o; }
 void oracle_set_threads(int t) {
 #ifdef _OPENMP

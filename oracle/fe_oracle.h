@@ -34,6 +34,8 @@ typedef struct oracle_ctx oracle_ctx;
 oracle_ctx* oracle_create(const fe_params* params, const float* xyz, uint32_t n, int fp64);
 void oracle_destroy(oracle_ctx* c);
 void oracle_set_params(oracle_ctx* c, const fe_params* params);
+/* Current parameter block (dt changes under the CFL flag). */
+void oracle_get_params(oracle_ctx* c, fe_params* out);
 void oracle_set_options(oracle_ctx* c, const fe_options* options);
 void oracle_set_threads(int threads);
 

@@ -40,6 +40,7 @@ def lib():
         "oracle_destroy": ([P], None),
         "oracle_set_params": ([P, c_void_p], None),
         "oracle_set_options": ([P, c_void_p], None),
+        "oracle_get_params": ([P, c_void_p], None),
         "oracle_set_threads": ([c_int], None),
         "oracle_upload": ([P, c_int, c_void_p, c_size_t], c_int),
         "oracle_download": ([P, c_int, c_void_p, c_size_t, c_int], c_int),
@@ -112,6 +113,12 @@ class Oracle:
     def set_params(self, params):
         self._params = params
         self._l.oracle_set_params(self._c, ctypes.addressof(params))
+
+    @property
+    def params(self):
+        p = type(self._params)()
+        self._l.oracle_get_params(self._c, ctypes.addressof(p))
+        return p
 
     def set_options(self, options):
         self._l.oracle_set_options(self._c, ctypes.addressof(options))

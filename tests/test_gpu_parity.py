@@ -1,0 +1,239 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle, stage by stage and
+frame by frame, from identical inputs.
+
+Bars: integer work (hashes, sort permutation, grid_start, the reorder gather, stats) is
+compared bit for bit. FP32 fields must satisfy |gpu - oracle| <= 1e-5 * max(|oracle|,
+rms(oracle field)) element-wise (the relative tolerance stated for this path, made robust
+for fields that cross zero).
+"""
+import numpy as np
+import pytest
+
+import fluid_engine as fe
+from fluid_engine import scene
+from fluid_engine._ffi import FeOptions
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+F = np.float32
+
+
+def opts(sort=fe.FE_SORT_RADIX, mode=fe.FE_SOLVER_FIXED, maxit=100, **kw):
+    o = fe.default_options()
+    o.sort_algorithm = sort
+    o.solver_mode = mode
+    o.max_iterations = maxit
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def assert_close(name, gpu, ref, tol=TOL):
+    gpu = np.asarray(gpu, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    assert gpu.shape == ref.shape, name
+    assert np.isfinite(gpu).all(), f"{name}: non-finite values"
+    rms = np.sqrt(np.mean(ref * ref)) if ref.size else 0.0
+    bound = tol * np.maximum(np.abs(ref), rms) + 1e-30
+    err = np.abs(gpu - ref)
+    bad = err > bound
+    assert not bad.any(), (f"{name}: {bad.sum()} of {bad.size} outside tolerance, "
+                           f"max err {err.max():.3e}, worst rel {(err / np.maximum(np.abs(ref), rms + 1e-30)).max():.3e}")
+
+
+def assert_exact(name, gpu, ref):
+    gpu = np.asarray(gpu)
+    ref = np.asarray(ref)
+    assert gpu.shape == ref.shape, name
+    if not np.array_equal(gpu.view(np.uint32) if gpu.dtype == np.float32 else gpu,
+                          ref.view(np.uint32) if ref.dtype == np.float32 else ref):
+        diff = np.nonzero(gpu.reshape(-1) != ref.reshape(-1))[0]
+        raise AssertionError(f"{name}: {len(diff)} mismatches, first at {diff[:5].tolist()}")
+
+
+def pair(w, options, oracle_mod):
+    sim = fe.Simulation(w.params, w.positions, device=0, options=options)
+    orc = oracle_mod.Oracle(w.params, w.positions, options=options)
+    return sim, orc
+
+
+def sync_fields(sim, orc, names):
+    """Copy the oracle's fields into the GPU so the next stage starts from identical input."""
+    for n in names:
+        sim.upload(n, orc.download(n))
+
+
+SCENES = {
+    "rest_density_4800": scene.rest_density_scene,
+    "dam_break_32k": lambda: scene.dam_break_32k(seed=1),
+}
+
+
+@pytest.fixture(scope="module", params=list(SCENES))
+def workload(request):
+    w = SCENES[request.param]()
+    print(f"[gpu parity] scene={w.name} n={w.n}", flush=True)
+    return w
+
+
+@pytest.mark.parametrize("sort", [fe.FE_SORT_RADIX, fe.FE_SORT_BITONIC])
+def test_neighbor_search_bit_exact(workload, sort, oracle_mod):
+    sim, orc = pair(workload, opts(sort=sort), oracle_mod)
+    sim.neighbor_search()
+    orc.neighbor_search()
+    for name in ("grid_entries", "grid_start", "position_b", "velocity_b"):
+        assert_exact(name, sim.download(name), orc.download(name))
+
+
+def test_stage_by_stage(workload, oracle_mod):
+    rng = np.random.default_rng(7)
+    sim, orc = pair(workload, opts(), oracle_mod)
+    n = workload.n
+    sim.neighbor_search()
+    orc.neighbor_search()
+    sim.density_alpha()
+    orc.density_alpha()
+    assert_close("densities", sim.download("densities"), orc.download("densities"))
+    assert_close("factors", sim.download("factors"), orc.download("factors"))
+    sync_fields(sim, orc, ["densities", "factors"])
+
+    v = np.zeros((n, 4), F)
+    v[:, :3] = rng.normal(0, 0.5, (n, 3))
+    for o in (sim, orc):
+        o.upload("velocity_b", v)
+        o.viscosity()
+    assert_close("viscosity velocity_a", sim.download("velocity_a"), orc.download("velocity_a"))
+    sync_fields(sim, orc, ["velocity_a"])
+
+    for o in (sim, orc):
+        o.density_source()
+    assert_close("density source_terms", sim.download("source_terms"), orc.download("source_terms"))
+    assert_exact("pressures zeroed", sim.download("pressures"), orc.download("pressures"))
+    sync_fields(sim, orc, ["source_terms"])
+
+    p = rng.uniform(0, 2000, n).astype(F)
+    for o in (sim, orc):
+        o.upload("pressures", p)
+        o.pressure_force()
+    assert_close("pressure_accelerations", sim.download("pressure_accelerations"),
+                 orc.download("pressure_accelerations"))
+    sync_fields(sim, orc, ["pressure_accelerations"])
+    for o in (sim, orc):
+        o.pressure_update()
+    assert_close("pressure_update pressures", sim.download("pressures"), orc.download("pressures"))
+    sync_fields(sim, orc, ["pressures"])
+
+    for o in (sim, orc):
+        o.pressure_integration()
+    assert_close("integrated position_a", sim.download("position_a"), orc.download("position_a"))
+    assert_close("integrated velocity_a", sim.download("velocity_a"), orc.download("velocity_a"))
+    sync_fields(sim, orc, ["position_a", "velocity_a"])
+
+    for o in (sim, orc):
+        o.neighbor_search()
+    for name in ("grid_entries", "grid_start", "position_b", "velocity_b"):
+        assert_exact("post-integrate " + name, sim.download(name), orc.download(name))
+    for o in (sim, orc):
+        o.density_alpha()
+    sync_fields(sim, orc, ["densities", "factors"])
+    for o in (sim, orc):
+        o.divergence_source()
+    assert_close("divergence source_terms", sim.download("source_terms"), orc.download("source_terms"))
+    sync_fields(sim, orc, ["source_terms"])
+    for o in (sim, orc):
+        o.divergence_integration()
+    assert_close("divergence velocity_a", sim.download("velocity_a"), orc.download("velocity_a"))
+    assert_close("colors", sim.download("colors"), orc.download("colors"))
+    sync_fields(sim, orc, ["velocity_a"])
+    for o in (sim, orc):
+        o.stats()
+    assert_exact("stats", sim.read_stats(), orc.read_stats())
+
+
+FIELDS_STEP = ("position_a", "velocity_a", "position_b", "velocity_b", "densities", "factors",
+               "source_terms", "pressures", "pressure_accelerations", "colors")
+
+
+@pytest.mark.parametrize("sort", [fe.FE_SORT_RADIX, fe.FE_SORT_BITONIC])
+def test_frames_match_oracle(workload, sort, oracle_mod):
+    sim, orc = pair(workload, opts(sort=sort), oracle_mod)
+    for frame in range(3):
+        sim.step()
+        orc.step()
+        assert_exact(f"frame {frame} grid_entries", sim.download("grid_entries"), orc.download("grid_entries"))
+        assert_exact(f"frame {frame} grid_start", sim.download("grid_start"), orc.download("grid_start"))
+        for name in FIELDS_STEP:
+            assert_close(f"frame {frame} {name}", sim.download(name), orc.download(name))
+        assert_exact(f"frame {frame} stats", sim.read_stats(), orc.read_stats())
+
+
+def test_converge_mode_iterations_match(oracle_mod):
+    w = scene.dam_break_32k(seed=2)
+    o = opts(mode=fe.FE_SOLVER_CONVERGE, maxit=100)
+    sim, orc = pair(w, o, oracle_mod)
+    for frame in range(3):
+        sim.step()
+        orc.step()
+        assert sim.last_iterations() == orc.last_iterations(), frame
+        for name in ("position_a", "velocity_a", "pressures"):
+            assert_close(f"converge frame {frame} {name}", sim.download(name), orc.download(name))
+    d, v, frames = sim.iteration_totals()
+    assert frames == 3 and d >= 3 and v >= 3
+
+
+def test_host_flags_cfl_and_early_exit(oracle_mod):
+    w = scene.rest_density_scene()
+    o = opts(use_cfl=1, use_solver_error_threshold=1)
+    sim, orc = pair(w, o, oracle_mod)
+    for frame in range(4):
+        sim.step(w.params.dt)
+        orc.step(w.params.dt)
+        assert sim.last_iterations() == orc.last_iterations(), frame
+        assert sim.params.dt == orc.params.dt, frame
+        assert_exact(f"flags frame {frame} stats", sim.read_stats(), orc.read_stats())
+        assert_close(f"flags frame {frame} position_a", sim.download("position_a"), orc.download("position_a"))
+
+
+@pytest.mark.parametrize("log2n", [20, 22])
+def test_radix_sort_random_keys_permutation(log2n):
+    n = 1 << log2n
+    print(f"[gpu parity] radix 2^{log2n}", flush=True)
+    rng = np.random.default_rng(log2n)
+    keys = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)
+    entries = np.stack([keys, np.arange(n, dtype=np.uint32)], axis=1)
+    sim = fe.Simulation(scene.make_params(0.01, 0.008, 0.04, 1000, 0.01, 0.5, 1e-3, 1, 1, (0, -9.81, 0),
+                                          (0, 0, 0), (1, 1, 1)), np.zeros((n, 3), F), device=0)
+    sim.upload("grid_entries", entries)
+    sim.sort()
+    out = sim.download("grid_entries")
+    perm = np.argsort(keys, kind="stable")
+    assert_exact("radix permutation", out[:, 1], perm.astype(np.uint32))
+    assert_exact("radix keys", out[:, 0], keys[perm])
+
+
+def test_bitonic_network_matches_oracle(oracle_mod):
+    n = 1 << 16
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, 4096, size=n).astype(np.uint32)   # many equal keys: order is the network's
+    entries = np.stack([keys, np.arange(n, dtype=np.uint32)], axis=1)
+    sim = fe.Simulation(scene.make_params(0.01, 0.008, 0.04, 1000, 0.01, 0.5, 1e-3, 1, 1, (0, -9.81, 0),
+                                          (0, 0, 0), (1, 1, 1)), np.zeros((n, 3), F), device=0,
+                        options=opts(sort=fe.FE_SORT_BITONIC))
+    sim.upload("grid_entries", entries)
+    sim.sort()
+    assert_exact("bitonic", sim.download("grid_entries"), oracle_mod.bitonic_sort_pairs(entries))
+
+
+def test_dam_break_1m_neighbor_search_and_density(oracle_mod):
+    w = scene.dam_break_1m(seed=0)
+    print(f"[gpu parity] {w.name} n={w.n}", flush=True)
+    sim, orc = pair(w, opts(), oracle_mod)
+    sim.neighbor_search()
+    orc.neighbor_search()
+    for name in ("grid_entries", "grid_start", "position_b"):
+        assert_exact("1M " + name, sim.download(name), orc.download(name))
+    sim.density_alpha()
+    orc.density_alpha()
+    assert_close("1M densities", sim.download("densities"), orc.download("densities"))
+    assert_close("1M factors", sim.download("factors"), orc.download("factors"))
