@@ -81,7 +81,9 @@ void free_ctx(fe_ctx* c) {
     for (void* p : {(void*)c->pos_a, (void*)c->pos_b, (void*)c->vel_a, (void*)c->vel_b, (void*)c->colors,
                     (void*)c->pacc, (void*)c->dens, (void*)c->fac, (void*)c->src, (void*)c->pres,
                     (void*)c->keys[0], (void*)c->keys[1], (void*)c->vals[0], (void*)c->vals[1],
-                    (void*)c->range, (void*)c->stats, (void*)c->resid, (void*)c->staging})
+                    (void*)c->range, (void*)c->stats, (void*)c->resid, (void*)c->staging,
+                    (void*)c->partial, (void*)c->stats_partial, (void*)c->nbr, (void*)c->nbr_count,
+                    (void*)c->sched[0], (void*)c->sched[1], (void*)c->mkeys[0], (void*)c->mkeys[1]})
         if (p) (void)hipFree(p);
     fe::sort_scratch_free(&c->sort);
     if (c->host_stats) (void)hipHostFree(c->host_stats);
@@ -108,6 +110,19 @@ void default_options(fe_options* o) {
     o->divergence_tol = 1e-2f;
     o->max_iterations = 100;
     o->warm_start = FE_WARM_START_REFERENCE;
+    o->arithmetic = FE_ARITH_FAST;
+}
+
+// Bits per axis of the Morton schedule key: enough for the box's cell extent (+ margin).
+uint32_t morton_bits_for(const fe_params& p) {
+    uint32_t bits = 1;
+    for (int a = 0; a < 3; a++) {
+        const float cells = (p.box_max[a] - p.box_min[a]) / p.smoothing_radius + 3.0f;
+        uint32_t b = 1;
+        while (b < 10 && (float)(1u << b) < cells) b++;
+        bits = b > bits ? b : bits;
+    }
+    return 3 * bits;
 }
 
 fe_status ensure_ends(fe_ctx* c) {
@@ -174,9 +189,18 @@ fe_status solve(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t* 
     const unsigned long long thr =
         (unsigned long long)((double)tol * (double)c->n * 4294967296.0);
     FE_HIP(fe::launch_fill_u64(c, slot, maxit), "fill residuals");
-    const uint32_t chunk = 8;
-    std::vector<hipEvent_t> events;
-    uint32_t launched = 0, checked = 0, found = 0;
+    // Chunk sizes: the previous frame's sweep count first (the usual answer), then 2, 4, 8,
+    // 16, ... Chunk j's residuals are read back while chunk j+1 runs.
+    struct Chunk {
+        hipEvent_t ev;
+        uint32_t beg, end;
+    };
+    std::vector<Chunk> chunks;
+    uint32_t first = c->last_iters[kind] ? c->last_iters[kind] : 4;
+    if (first > maxit) first = maxit;
+    uint32_t chunk_beg = 0, chunk_end = first, grow = 2;
+    uint32_t launched = 0, found = 0;
+    size_t checked_chunks = 0;
     bool done = false;
     fe_status st = FE_OK;
     for (uint32_t k = 0; k < maxit && !done; k++) {
@@ -185,37 +209,36 @@ fe_status solve(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t* 
         if (e == hipSuccess) e = fe::launch_pressure_update(c, slot + k, prev, thr);
         if (e != hipSuccess) { st = hip_fail(e, "jacobi sweep"); break; }
         launched = k + 1;
-        if (launched % chunk == 0 || launched == maxit) {
-            const uint32_t beg = launched - ((launched - 1) % chunk + 1);
-            hipEvent_t ev;
-            e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(hslot + beg, slot + beg, sizeof(unsigned long long) * (launched - beg),
-                                   hipMemcpyDeviceToHost, c->stream);
-            if (e == hipSuccess) e = hipEventRecord(ev, c->stream);
-            if (e != hipSuccess) { st = hip_fail(e, "residual readback"); break; }
-            events.push_back(ev);
-            // look at the chunk before this one (already finished or nearly so)
-            if (events.size() >= 2) {
-                e = hipEventSynchronize(events[events.size() - 2]);
-                if (e != hipSuccess) { st = hip_fail(e, "residual wait"); break; }
-                const uint32_t end = beg;
-                for (uint32_t q = checked; q < end; q++) {
-                    if (hslot[q] <= thr) { found = q + 1; done = true; break; }
-                }
-                checked = end;
-            }
+        if (launched < chunk_end && launched < maxit) continue;
+        hipEvent_t ev;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(hslot + chunk_beg, slot + chunk_beg,
+                               sizeof(unsigned long long) * (launched - chunk_beg), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev, c->stream);
+        if (e != hipSuccess) { st = hip_fail(e, "residual readback"); break; }
+        chunks.push_back(Chunk{ev, chunk_beg, launched});
+        chunk_beg = launched;
+        chunk_end = launched + grow;
+        grow = grow < 16 ? grow * 2 : 16;
+        // look at the chunk before this one (finished or nearly so)
+        if (chunks.size() >= 2) {
+            const Chunk& ch = chunks[chunks.size() - 2];
+            e = hipEventSynchronize(ch.ev);
+            if (e != hipSuccess) { st = hip_fail(e, "residual wait"); break; }
+            for (uint32_t q = ch.beg; q < ch.end; q++)
+                if (hslot[q] <= thr) { found = q + 1; done = true; break; }
+            checked_chunks = chunks.size() - 1;
         }
     }
-    if (st == FE_OK && !events.empty()) {
-        hipError_t e = hipEventSynchronize(events.back());
+    if (st == FE_OK && !chunks.empty()) {
+        hipError_t e = hipEventSynchronize(chunks.back().ev);
         if (e != hipSuccess) st = hip_fail(e, "residual wait");
-        if (st == FE_OK && !found) {
-            for (uint32_t q = checked; q < launched; q++)
+        for (size_t j = checked_chunks; st == FE_OK && !found && j < chunks.size(); j++)
+            for (uint32_t q = chunks[j].beg; q < chunks[j].end; q++)
                 if (hslot[q] <= thr) { found = q + 1; break; }
-        }
     }
-    for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
+    for (const Chunk& ch : chunks) (void)hipEventDestroy(ch.ev);
     if (st != FE_OK) return st;
     const uint32_t its = found ? found : launched;
     c->last_iters[kind] = its;
@@ -235,6 +258,10 @@ fe_status neighbor_search(fe_ctx* c) {
     }
     FE_HIP(fe::launch_offsets_reorder(c), "offsets/reorder");
     c->ends_dirty = false;
+    c->list_valid = false;
+    // gather schedule: particles in Morton order of their cells
+    FE_HIP(fe::launch_morton_keys(c), "morton keys");
+    FE_HIP(fe::radix_sort_pairs(c, c->mkeys, c->sched, c->n, c->morton_bits, "schedule"), "schedule sort");
     return FE_OK;
 }
 
@@ -350,6 +377,7 @@ fe_status fe_create(const fe_params* params, const float* xyz, uint32_t n, int d
     fe_ctx* c = new fe_ctx();
     c->device = device;
     c->params = *params;
+    c->morton_bits = morton_bits_for(*params);
     default_options(&c->options);
     c->n = n;
     c->T = 1;
@@ -376,6 +404,14 @@ fe_status fe_create(const fe_params* params, const float* xyz, uint32_t n, int d
     ALLOC(c->range, 2 * tb);
     ALLOC(c->stats, 16);
     ALLOC(c->resid, sizeof(unsigned long long) * 2 * fe::kMaxSolveIters);
+    ALLOC(c->partial, sizeof(unsigned long long) * fe::gather_partials(n));
+    ALLOC(c->stats_partial, sizeof(uint32_t) * 3 * fe::kStatsBlocks);
+    ALLOC(c->nbr, sizeof(uint32_t) * fe::kMaxNbr * (size_t)n);
+    ALLOC(c->nbr_count, sizeof(uint32_t) * (size_t)n);
+    for (int k = 0; k < 2; k++) {
+        ALLOC(c->sched[k], sizeof(uint32_t) * (size_t)n);
+        ALLOC(c->mkeys[k], sizeof(uint32_t) * (size_t)n);
+    }
 #undef ALLOC
     e = fe::sort_scratch_alloc(&c->sort, c->T);
     if (e != hipSuccess) { free_ctx(c); return hip_fail(e, "sort scratch"); }
@@ -405,6 +441,7 @@ fe_status fe_create(const fe_params* params, const float* xyz, uint32_t n, int d
     if (e == hipSuccess) e = hipMemset(c->range, 0xFF, 2 * tb);
     if (e == hipSuccess) e = hipMemset(c->keys[0], 0xFF, tb);
     if (e == hipSuccess) e = hipMemset(c->vals[0], 0xFF, tb);
+    if (e == hipSuccess) e = fe::launch_iota(c, c->sched[0], n);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) { free_ctx(c); return hip_fail(e, "initial upload"); }
     std::memset(c->host_stats, 0, 16);
@@ -425,6 +462,7 @@ fe_status fe_set_params(fe_ctx* c, const fe_params* p) {
     if (!p) return fail(FE_ERR_INVALID_ARGUMENT, "null params");
     if (!(p->smoothing_radius > 0.0f)) return fail(FE_ERR_INVALID_ARGUMENT, "smoothing_radius must be > 0");
     c->params = *p;
+    c->morton_bits = morton_bits_for(*p);
     return FE_OK;
 }
 
@@ -442,6 +480,8 @@ fe_status fe_set_options(fe_ctx* c, const fe_options* o) {
         return fail(FE_ERR_INVALID_ARGUMENT, "unknown sort_algorithm");
     if (o->solver_mode != FE_SOLVER_FIXED && o->solver_mode != FE_SOLVER_CONVERGE)
         return fail(FE_ERR_INVALID_ARGUMENT, "unknown solver_mode");
+    if (o->arithmetic != FE_ARITH_FAST && o->arithmetic != FE_ARITH_REFERENCE)
+        return fail(FE_ERR_INVALID_ARGUMENT, "unknown arithmetic");
     if (o->warm_start != FE_WARM_START_REFERENCE)
         return fail(FE_ERR_UNSUPPORTED, "warm_start mode not available in this build");
     if (o->solver_mode == FE_SOLVER_CONVERGE && (o->max_iterations < 1 || o->max_iterations > fe::kMaxSolveIters))
@@ -473,6 +513,8 @@ fe_status fe_upload(fe_ctx* c, int field, const void* src, size_t bytes) {
         return fail(FE_ERR_SIZE_MISMATCH, "field size mismatch: expected " + std::to_string(want) + " bytes");
     if (!src) return fail(FE_ERR_INVALID_ARGUMENT, "null source");
     DeviceGuard g(c->device);
+    if (field == FE_FIELD_POSITION_B || field == FE_FIELD_GRID_ENTRIES || field == FE_FIELD_GRID_START)
+        c->list_valid = false;
     if (dev) {
         FE_HIP(hipMemcpyAsync(dev, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
         FE_HIP(hipStreamSynchronize(c->stream), "upload wait");
@@ -544,6 +586,7 @@ FE_STAGE(fe_sort, {
     else
         FE_HIP(fe::radix_sort(c, c->T, 32), "radix sort");
     c->ends_dirty = true;
+    c->list_valid = false;
 })
 FE_STAGE(fe_density_alpha, {
     fe_status s = ensure_ends(c);

@@ -1,0 +1,81 @@
+// fe_device.h — device helpers shared by the HIP kernels: the SPH constants, the cell hash
+// (common.glsl:30-35) and the cubic-spline kernel (common.glsl:38-73).
+#pragma once
+
+#include "fe_internal.h"
+
+namespace fe {
+
+constexpr float kPi = 3.1415926535897932384626433832795f;
+constexpr uint32_t kP1 = 73856093u, kP2 = 19349663u, kP3 = 83492791u;
+
+struct Sph {
+    float h, hh, mass, k;   // k = 8 / (pi h^3) of common.glsl:42-43 / :61-62
+    float inv_h;            // 1 / h: the cell coordinate is floor(pos * (1 / h))
+    uint32_t n, mask;       // particles, T - 1 (T is a power of two, so % T == & (T - 1))
+};
+
+inline Sph sph_of(const fe_ctx* c) {
+    Sph s;
+    s.h = c->params.smoothing_radius;
+    s.hh = s.h * s.h;
+    s.mass = c->params.particle_mass;
+    const float h3 = s.h * s.h * s.h;
+    s.k = 8.0f / (kPi * h3);
+    s.inv_h = 1.0f / s.h;
+    s.n = c->n;
+    s.mask = c->T - 1u;
+    return s;
+}
+
+__device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, uint32_t mask) {
+    return ((uint32_t)cx * kP1 ^ (uint32_t)cy * kP2 ^ (uint32_t)cz * kP3) & mask;
+}
+
+// ivec3(floor(pos / h)) evaluated as floor(pos * (1 / h)), the form the reference GPU ran.
+__device__ __forceinline__ int3 cell_of(float x, float y, float z, float inv_h) {
+    return make_int3((int)floorf(x * inv_h), (int)floorf(y * inv_h), (int)floorf(z * inv_h));
+}
+
+// Cubic spline of common.glsl:38-54 for an accepted pair (r <= h). The quotient r / h is
+// taken as r * (1 / h) (the form the reference GPU used for pos / h).
+__device__ __forceinline__ float spline_w(float r, const Sph& s) {
+    const float q = r * s.inv_h;
+    if (q >= 1.0f) return 0.0f;
+    if (q <= 0.5f) {
+        const float q2 = q * q;
+        return s.k * (6.0f * (q2 * q) - 6.0f * q2 + 1.0f);
+    }
+    const float omq = 1.0f - q;
+    return s.k * 2.0f * (omq * omq * omq);
+}
+
+// common.glsl:56-73: the scalar g with grad W = g * r_vec (zero for q >= 1 or r < 1e-6).
+__device__ __forceinline__ float spline_grad(float r, const Sph& s) {
+    const float q = r * s.inv_h;
+    if (q >= 1.0f || r < 1e-6f) return 0.0f;
+    float gf;
+    if (q <= 0.5f) {
+        gf = s.k * (18.0f * q * q - 12.0f * q);
+    } else {
+        const float omq = 1.0f - q;
+        gf = s.k * -6.0f * omq * omq;
+    }
+    return gf * __builtin_amdgcn_rcpf(s.h * r);
+}
+
+// Quantised compression residual of one particle (FE_SOLVER_CONVERGE):
+// q = floor(min(max(0, -(s - Ap)) * dt / rho0, 256) * 2^32); integer sums make the decision
+// independent of summation order.
+__device__ __forceinline__ unsigned long long residual_q(float s_minus_ap, float dt, float rho0) {
+    const float neg = -s_minus_ap;
+    if (!(neg > 0.0f)) return 0ull;
+    float e = (neg * dt) / rho0;
+    if (!(e < 256.0f)) e = 256.0f;
+    return (unsigned long long)(e * 4294967296.0f);
+}
+
+// Hardware square root (1 ulp) for pair distances.
+__device__ __forceinline__ float pair_sqrt(float r2) { return __builtin_amdgcn_sqrtf(r2); }
+
+}  // namespace fe

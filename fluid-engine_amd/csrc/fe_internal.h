@@ -15,6 +15,8 @@ namespace fe {
 constexpr uint32_t kSentinel = 0xFFFFFFFFu;
 constexpr uint32_t kBlock = 256;          // threads per block of the per-particle kernels
 constexpr uint32_t kMaxSolveIters = 1024; // residual slots per solve
+constexpr uint32_t kStatsBlocks = 1024;   // blocks of the stats reduction
+constexpr uint32_t kMaxNbr = 64;          // neighbour-list slots per particle
 
 // Per-kernel event profiling (enabled by fe_profile_enable).
 struct ProfileRecord {
@@ -68,12 +70,24 @@ struct fe_ctx {
     // grid_start (x) and one-past-the-end of each bucket (y); x = 0xFFFFFFFF when empty.
     uint2* range = nullptr;
     bool ends_dirty = false;   // y must be rebuilt from x and the keys (after uploads)
+    // neighbour list of the current (pos_b, range): slot-major nbr[k * n + i], k < kMaxNbr,
+    // and the pair count of each particle (which may exceed kMaxNbr)
+    uint32_t* nbr = nullptr;
+    uint32_t* nbr_count = nullptr;
+    bool list_valid = false;
+    // work schedule of the solver gathers: particle indices ordered by the Morton code of
+    // their cell (sched[0] holds it; [1] is the sort's partner), and the Morton keys
+    uint32_t* sched[2] = {nullptr, nullptr};
+    uint32_t* mkeys[2] = {nullptr, nullptr};
+    uint32_t morton_bits = 30;   // bits of the Morton key (3 x bits per axis)
 
     fe::SortScratch sort;
 
     // stats_buffer and the convergence residuals (one slot per sweep, two solves)
     uint32_t* stats = nullptr;
     unsigned long long* resid = nullptr;   // 2 * kMaxSolveIters
+    unsigned long long* partial = nullptr; // per-wave residual partials of one sweep
+    uint32_t* stats_partial = nullptr;     // 3 x kStatsBlocks per-block stats partials
     uint32_t* host_stats = nullptr;         // pinned [3]
     unsigned long long* host_resid = nullptr;  // pinned 2 * kMaxSolveIters
     hipEvent_t stats_event = nullptr;
@@ -111,14 +125,34 @@ struct Launch {
 // Stable LSD radix sort of n (key, value) pairs on bits [0, key_bits) of the key.
 // The result is in keys[0]/vals[0] on return (buffers are swapped when needed).
 hipError_t radix_sort(fe_ctx* c, uint32_t n, uint32_t key_bits);
+// The same sort on an explicit ping-pong pair of buffers (result in keys[0]/vals[0]).
+hipError_t radix_sort_pairs(fe_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
+                            const char* tag);
 // The bitonic network of bitonic_sort.comp over all T entries, in place in keys[0]/vals[0].
 hipError_t bitonic_sort(fe_ctx* c);
 hipError_t sort_scratch_alloc(SortScratch* s, uint32_t max_n);
 void sort_scratch_free(SortScratch* s);
 
+// ---- launchers (fe_gather.hip) ----
+// Number of per-wave residual partials the pressure_update gather writes for n particles.
+uint32_t gather_partials(uint32_t n);
+
+// ---- reference-arithmetic gathers (fe_exact.hip) ----
+hipError_t exact_density_alpha(fe_ctx* c);
+hipError_t exact_viscosity(fe_ctx* c);
+hipError_t exact_density_source(fe_ctx* c);
+hipError_t exact_divergence_source(fe_ctx* c);
+hipError_t exact_pressure_force(fe_ctx* c, const unsigned long long* prev, unsigned long long thr);
+// Writes one residual partial per block into `partial` (when set); *partial_count = blocks.
+hipError_t exact_pressure_update(fe_ctx* c, unsigned long long* partial, const unsigned long long* prev,
+                                 unsigned long long thr, uint32_t* partial_count);
+
 // ---- launchers (fe_sph.hip) ----
 hipError_t launch_hash(fe_ctx* c);
 hipError_t launch_offsets_reorder(fe_ctx* c);
+// Morton keys of the cells of position_b and the identity schedule payload.
+hipError_t launch_morton_keys(fe_ctx* c);
+hipError_t launch_iota(fe_ctx* c, uint32_t* p, uint32_t count);
 hipError_t launch_fill_ranges(fe_ctx* c);
 hipError_t launch_offsets_only(fe_ctx* c);
 hipError_t launch_rebuild_ends(fe_ctx* c);

@@ -267,17 +267,19 @@ void sort_scratch_free(SortScratch* s) {
     s->counts = s->totals = nullptr;
 }
 
-hipError_t radix_sort(fe_ctx* c, uint32_t n, uint32_t key_bits) {
+hipError_t radix_sort_pairs(fe_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
+                            const char* tag) {
     if (n <= 1) return hipSuccess;
     const uint32_t tiles = (n + kTile - 1) / kTile;
     if (tiles > c->sort.max_tiles) return hipErrorInvalidValue;
     const uint32_t passes = (key_bits + 7) / 8;
+    (void)tag;
     for (uint32_t p = 0; p < passes; p++) {
         const uint32_t shift = 8 * p;
         {
             Launch L(c, "radix_hist");
-            hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, c->stream, c->keys[0], n, shift,
-                               tiles, c->sort.counts);
+            hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, c->stream, keys[0], n, shift, tiles,
+                               c->sort.counts);
         }
         {
             Launch L(c, "radix_scan");
@@ -286,14 +288,17 @@ hipError_t radix_sort(fe_ctx* c, uint32_t n, uint32_t key_bits) {
         }
         {
             Launch L(c, "radix_scatter");
-            hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, c->stream, c->keys[0],
-                               c->vals[0], c->keys[1], c->vals[1], n, shift, tiles, c->sort.counts,
-                               c->sort.totals);
+            hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, c->stream, keys[0], vals[0],
+                               keys[1], vals[1], n, shift, tiles, c->sort.counts, c->sort.totals);
         }
-        std::swap(c->keys[0], c->keys[1]);
-        std::swap(c->vals[0], c->vals[1]);
+        std::swap(keys[0], keys[1]);
+        std::swap(vals[0], vals[1]);
     }
     return hipGetLastError();
+}
+
+hipError_t radix_sort(fe_ctx* c, uint32_t n, uint32_t key_bits) {
+    return radix_sort_pairs(c, c->keys, c->vals, n, key_bits, "entries");
 }
 
 hipError_t bitonic_sort(fe_ctx* c) {
