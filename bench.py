@@ -50,8 +50,6 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="dam_break_1m", choices=["dam_break_1m", "dam_break_32k", "block_4m"])
     ap.add_argument("--solver", default="converge", choices=["converge", "fixed"])
-    ap.add_argument("--arith", default="reference", choices=["reference", "fast"],
-                    help="gather arithmetic: reference order + IEEE (bit-exact) or the LDS cell-group path")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -76,7 +74,6 @@ def main():
     opts.density_tol = 1e-3
     opts.divergence_tol = 1e-2
     opts.max_iterations = 100
-    opts.arithmetic = fe.FE_ARITH_REFERENCE if args.arith == "reference" else fe.FE_ARITH_FAST
     print(f"[bench] {w.name} n={w.n} solver={args.solver} warmup={args.warmup} steps={args.steps}",
           file=sys.stderr, flush=True)
     sim = fe.Simulation(w.params, w.positions, device=0, options=opts)
@@ -102,7 +99,7 @@ def main():
     sim.profile(False)
     kernels = {}
     for name, (launches, total_ms) in prof.items():
-        per = ALGO_BYTES.get(name.removesuffix("_ref"))
+        per = ALGO_BYTES.get(name)
         avg_us = total_ms * 1e3 / max(launches, 1)
         gbs = per * w.n / (avg_us * 1e-6) / 1e9 if per else None
         kernels[name] = {"launches_per_step": launches / args.profile_steps, "avg_us": round(avg_us, 3),
@@ -110,9 +107,9 @@ def main():
     tot = sum(v[1] for v in prof.values())
     for name in kernels:
         kernels[name]["share"] = round(prof[name][1] / tot, 4) if tot else None
-    dom = max((k for k in kernels if ALGO_BYTES.get(k.removesuffix("_ref"))), key=lambda k: prof[k][1])
+    dom = max((k for k in kernels if ALGO_BYTES.get(k)), key=lambda k: prof[k][1])
     dom_avg_s = kernels[dom]["avg_us"] * 1e-6
-    dom_bytes = ALGO_BYTES[dom.removesuffix("_ref")]
+    dom_bytes = ALGO_BYTES[dom]
     achieved = dom_bytes * w.n / dom_avg_s / 1e9
     print("[bench] kernels: " + json.dumps(kernels), file=sys.stderr, flush=True)
 
@@ -135,7 +132,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded jittered block)",
         "config": {"workload": w.name, "particles": w.n, "table_size": sim.table_size, "dt": dt,
-                   "solver": args.solver, "arithmetic": args.arith, "density_tol": 1e-3, "divergence_tol": 1e-2, "max_iterations": 100,
+                   "solver": args.solver, "density_tol": 1e-3, "divergence_tol": 1e-2, "max_iterations": 100,
                    "sort": "radix"},
         "particle_updates_per_s": round(steps_per_s * w.n, 1),
         "jacobi_sweeps_per_step": {"density": round(d_it / max(frames, 1), 3),

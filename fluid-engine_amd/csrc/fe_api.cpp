@@ -83,7 +83,9 @@ void free_ctx(fe_ctx* c) {
                     (void*)c->keys[0], (void*)c->keys[1], (void*)c->vals[0], (void*)c->vals[1],
                     (void*)c->range, (void*)c->stats, (void*)c->resid, (void*)c->staging,
                     (void*)c->partial, (void*)c->stats_partial, (void*)c->nbr, (void*)c->nbr_count,
-                    (void*)c->sched[0], (void*)c->sched[1], (void*)c->mkeys[0], (void*)c->mkeys[1]})
+                    (void*)c->sched[0], (void*)c->sched[1], (void*)c->mkeys[0], (void*)c->mkeys[1],
+                    (void*)c->inv, (void*)c->pos_m, (void*)c->vb_m, (void*)c->va_m, (void*)c->dens_m,
+                    (void*)c->prho_m, (void*)c->pacc_m})
         if (p) (void)hipFree(p);
     fe::sort_scratch_free(&c->sort);
     if (c->host_stats) (void)hipHostFree(c->host_stats);
@@ -110,7 +112,6 @@ void default_options(fe_options* o) {
     o->divergence_tol = 1e-2f;
     o->max_iterations = 100;
     o->warm_start = FE_WARM_START_REFERENCE;
-    o->arithmetic = FE_ARITH_FAST;
 }
 
 // Bits per axis of the Morton schedule key: enough for the box's cell extent (+ margin).
@@ -262,6 +263,7 @@ fe_status neighbor_search(fe_ctx* c) {
     // gather schedule: particles in Morton order of their cells
     FE_HIP(fe::launch_morton_keys(c), "morton keys");
     FE_HIP(fe::radix_sort_pairs(c, c->mkeys, c->sched, c->n, c->morton_bits, "schedule"), "schedule sort");
+    FE_HIP(fe::launch_schedule_mirrors(c), "schedule mirrors");
     return FE_OK;
 }
 
@@ -412,6 +414,8 @@ fe_status fe_create(const fe_params* params, const float* xyz, uint32_t n, int d
         ALLOC(c->sched[k], sizeof(uint32_t) * (size_t)n);
         ALLOC(c->mkeys[k], sizeof(uint32_t) * (size_t)n);
     }
+    ALLOC(c->inv, s1); ALLOC(c->pos_m, v4); ALLOC(c->vb_m, v4); ALLOC(c->va_m, v4);
+    ALLOC(c->dens_m, s1); ALLOC(c->prho_m, s1); ALLOC(c->pacc_m, v4);
 #undef ALLOC
     e = fe::sort_scratch_alloc(&c->sort, c->T);
     if (e != hipSuccess) { free_ctx(c); return hip_fail(e, "sort scratch"); }
@@ -480,8 +484,6 @@ fe_status fe_set_options(fe_ctx* c, const fe_options* o) {
         return fail(FE_ERR_INVALID_ARGUMENT, "unknown sort_algorithm");
     if (o->solver_mode != FE_SOLVER_FIXED && o->solver_mode != FE_SOLVER_CONVERGE)
         return fail(FE_ERR_INVALID_ARGUMENT, "unknown solver_mode");
-    if (o->arithmetic != FE_ARITH_FAST && o->arithmetic != FE_ARITH_REFERENCE)
-        return fail(FE_ERR_INVALID_ARGUMENT, "unknown arithmetic");
     if (o->warm_start != FE_WARM_START_REFERENCE)
         return fail(FE_ERR_UNSUPPORTED, "warm_start mode not available in this build");
     if (o->solver_mode == FE_SOLVER_CONVERGE && (o->max_iterations < 1 || o->max_iterations > fe::kMaxSolveIters))
@@ -515,6 +517,16 @@ fe_status fe_upload(fe_ctx* c, int field, const void* src, size_t bytes) {
     DeviceGuard g(c->device);
     if (field == FE_FIELD_POSITION_B || field == FE_FIELD_GRID_ENTRIES || field == FE_FIELD_GRID_START)
         c->list_valid = false;
+    // schedule-ordered mirrors of the uploaded field are stale now
+    switch (field) {
+        case FE_FIELD_POSITION_B: c->mirrors &= ~fe::kMirrorPos; break;
+        case FE_FIELD_VELOCITY_B: c->mirrors &= ~fe::kMirrorVelB; break;
+        case FE_FIELD_VELOCITY_A: c->mirrors &= ~fe::kMirrorVelA; break;
+        case FE_FIELD_DENSITIES: c->mirrors &= ~(fe::kMirrorDens | fe::kMirrorPrho); break;
+        case FE_FIELD_PRESSURES: c->mirrors &= ~fe::kMirrorPrho; break;
+        case FE_FIELD_PRESSURE_ACCELERATIONS: c->mirrors &= ~fe::kMirrorPacc; break;
+        default: break;
+    }
     if (dev) {
         FE_HIP(hipMemcpyAsync(dev, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
         FE_HIP(hipStreamSynchronize(c->stream), "upload wait");

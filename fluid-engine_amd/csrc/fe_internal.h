@@ -17,6 +17,9 @@ constexpr uint32_t kBlock = 256;          // threads per block of the per-partic
 constexpr uint32_t kMaxSolveIters = 1024; // residual slots per solve
 constexpr uint32_t kStatsBlocks = 1024;   // blocks of the stats reduction
 constexpr uint32_t kMaxNbr = 64;          // neighbour-list slots per particle
+// schedule-ordered mirrors of the fields the gathers read from neighbours
+constexpr uint32_t kMirrorPos = 1u, kMirrorVelB = 2u, kMirrorVelA = 4u, kMirrorDens = 8u, kMirrorPrho = 16u,
+                   kMirrorPacc = 32u, kMirrorAll = 63u;
 
 // Per-kernel event profiling (enabled by fe_profile_enable).
 struct ProfileRecord {
@@ -70,8 +73,8 @@ struct fe_ctx {
     // grid_start (x) and one-past-the-end of each bucket (y); x = 0xFFFFFFFF when empty.
     uint2* range = nullptr;
     bool ends_dirty = false;   // y must be rebuilt from x and the keys (after uploads)
-    // neighbour list of the current (pos_b, range): slot-major nbr[k * n + i], k < kMaxNbr,
-    // and the pair count of each particle (which may exceed kMaxNbr)
+    // neighbour list of the current (pos_b, range): one row of kMaxNbr entries per work-item
+    // of the gather schedule, and the pair count of each (which may exceed kMaxNbr)
     uint32_t* nbr = nullptr;
     uint32_t* nbr_count = nullptr;
     bool list_valid = false;
@@ -80,6 +83,14 @@ struct fe_ctx {
     uint32_t* sched[2] = {nullptr, nullptr};
     uint32_t* mkeys[2] = {nullptr, nullptr};
     uint32_t morton_bits = 30;   // bits of the Morton key (3 x bits per axis)
+    uint32_t* inv = nullptr;     // inverse schedule: hash index -> schedule position
+    float4* pos_m = nullptr;     // position_b by schedule position
+    float4* vb_m = nullptr;      // velocity_b
+    float4* va_m = nullptr;      // velocity_a
+    float* dens_m = nullptr;     // densities
+    float* prho_m = nullptr;     // pressure / density^2
+    float4* pacc_m = nullptr;    // pressure_accelerations
+    uint32_t mirrors = 0;        // kMirror* bits that are current
 
     fe::SortScratch sort;
 
@@ -134,18 +145,10 @@ hipError_t sort_scratch_alloc(SortScratch* s, uint32_t max_n);
 void sort_scratch_free(SortScratch* s);
 
 // ---- launchers (fe_gather.hip) ----
-// Number of per-wave residual partials the pressure_update gather writes for n particles.
+// Number of per-block residual partials the pressure_update gather writes for n particles.
 uint32_t gather_partials(uint32_t n);
-
-// ---- reference-arithmetic gathers (fe_exact.hip) ----
-hipError_t exact_density_alpha(fe_ctx* c);
-hipError_t exact_viscosity(fe_ctx* c);
-hipError_t exact_density_source(fe_ctx* c);
-hipError_t exact_divergence_source(fe_ctx* c);
-hipError_t exact_pressure_force(fe_ctx* c, const unsigned long long* prev, unsigned long long thr);
-// Writes one residual partial per block into `partial` (when set); *partial_count = blocks.
-hipError_t exact_pressure_update(fe_ctx* c, unsigned long long* partial, const unsigned long long* prev,
-                                 unsigned long long thr, uint32_t* partial_count);
+// Inverse schedule and the position / velocity_b mirrors after the schedule sort.
+hipError_t launch_schedule_mirrors(fe_ctx* c);
 
 // ---- launchers (fe_sph.hip) ----
 hipError_t launch_hash(fe_ctx* c);

@@ -348,6 +348,7 @@ hipError_t launch_rebuild_ends(fe_ctx* c) {
 
 
 hipError_t launch_pressure_integration(fe_ctx* c) {
+    c->mirrors &= ~kMirrorVelA;
     Launch L(c, "pressure_integration");
     const fe_params& p = c->params;
     hipLaunchKernelGGL(k_pressure_integration, grid_for(c->n), dim3(kBlock), 0, c->stream, c->n, c->pos_b, c->pacc,
@@ -359,6 +360,7 @@ hipError_t launch_pressure_integration(fe_ctx* c) {
 
 
 hipError_t launch_divergence_integration(fe_ctx* c) {
+    c->mirrors &= ~kMirrorVelA;
     Launch L(c, "divergence_integration");
     hipLaunchKernelGGL(k_divergence_integration, grid_for(c->n), dim3(kBlock), 0, c->stream, c->n, c->vel_a, c->pacc,
                        c->colors, c->params.dt);

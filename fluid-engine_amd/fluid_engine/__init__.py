@@ -14,14 +14,14 @@ from ctypes import byref, c_double, c_float, c_uint32, c_uint64, c_void_p
 import numpy as np
 
 from . import scene
-from ._ffi import (FE_ARITH_FAST, FE_ARITH_REFERENCE, EXPORTED_SYMBOLS, FE_SOLVER_CONVERGE, FE_SOLVER_FIXED, FE_SORT_BITONIC, FE_SORT_RADIX,
+from ._ffi import (EXPORTED_SYMBOLS, FE_SOLVER_CONVERGE, FE_SOLVER_FIXED, FE_SORT_BITONIC, FE_SORT_RADIX,
                    FE_WARM_START_LEARNED, FE_WARM_START_REFERENCE, FIELDS, INT_FIELDS, LIB_PATH, FeOptions,
                    FeParams, FluidEngineError, check, load_library)
 
 __all__ = [
     "Simulation", "FeParams", "FeOptions", "FluidEngineError", "load_library", "scene", "frame_control",
     "default_options", "FE_SORT_BITONIC", "FE_SORT_RADIX", "FE_SOLVER_FIXED", "FE_SOLVER_CONVERGE",
-    "FE_WARM_START_REFERENCE", "FE_WARM_START_LEARNED", "FE_ARITH_FAST", "FE_ARITH_REFERENCE", "FIELDS", "EXPORTED_SYMBOLS", "LIB_PATH",
+    "FE_WARM_START_REFERENCE", "FE_WARM_START_LEARNED", "FIELDS", "EXPORTED_SYMBOLS", "LIB_PATH",
 ]
 
 

@@ -21,8 +21,6 @@ FE_SOLVER_FIXED = 0
 FE_SOLVER_CONVERGE = 1
 FE_WARM_START_REFERENCE = 0
 FE_WARM_START_LEARNED = 1
-FE_ARITH_FAST = 0
-FE_ARITH_REFERENCE = 1
 
 # field name -> (id, floats-or-words per particle, per-table (True) or per-particle)
 FIELDS = {
@@ -75,8 +73,7 @@ class FeOptions(Structure):
         ("divergence_tol", c_float),
         ("max_iterations", c_uint32),
         ("warm_start", c_uint32),
-        ("arithmetic", c_uint32),
-        ("reserved", c_uint32 * 5),
+        ("reserved", c_uint32 * 6),
     ]
 
 

@@ -79,15 +79,6 @@ typedef struct fe_params {
 #define FE_WARM_START_REFERENCE 0u
 #define FE_WARM_START_LEARNED 1u
 
-/* Arithmetic of the neighbour gathers. FAST (default): cell-group gathers with candidates
- * staged in LDS, per-particle sums split over several lanes, hardware square root and
- * reciprocal; every stage agrees with the reference arithmetic to ~1e-7 relative from
- * identical inputs. REFERENCE: one work-item per particle in the reference visit order with
- * IEEE division and square root (the shaders' FP32 operation sequence), for bit-level
- * reproducibility of whole frames. */
-#define FE_ARITH_FAST 0u
-#define FE_ARITH_REFERENCE 1u
-
 /* Run options. The first five mirror the runtime flags of `AppUI`
  * (src/renderer/ui.rs:12-49): sort_algorithm, use_cfl, use_solver_error_threshold,
  * density_error_pct, divergence_error_pct, whose host logic is in Renderer::step
@@ -103,8 +94,7 @@ typedef struct fe_options {
     float divergence_tol;                /* 1e-2 (relative) for FE_SOLVER_CONVERGE */
     uint32_t max_iterations;             /* 100 */
     uint32_t warm_start;                 /* FE_WARM_START_REFERENCE */
-    uint32_t arithmetic;                 /* FE_ARITH_FAST */
-    uint32_t reserved[5];
+    uint32_t reserved[6];
 } fe_options;
 
 /* Device buffers of `GpuPhysicsData` (src/entities/particle.rs:137-161).

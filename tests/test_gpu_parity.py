@@ -19,12 +19,11 @@ TOL = 1e-5
 F = np.float32
 
 
-def opts(sort=fe.FE_SORT_RADIX, mode=fe.FE_SOLVER_FIXED, maxit=100, arith=fe.FE_ARITH_FAST, **kw):
+def opts(sort=fe.FE_SORT_RADIX, mode=fe.FE_SOLVER_FIXED, maxit=100, **kw):
     o = fe.default_options()
     o.sort_algorithm = sort
     o.solver_mode = mode
     o.max_iterations = maxit
-    o.arithmetic = arith
     for k, v in kw.items():
         setattr(o, k, v)
     return o
@@ -51,6 +50,11 @@ def assert_exact(name, gpu, ref):
                           ref.view(np.uint32) if ref.dtype == np.float32 else ref):
         diff = np.nonzero(gpu.reshape(-1) != ref.reshape(-1))[0]
         raise AssertionError(f"{name}: {len(diff)} mismatches, first at {diff[:5].tolist()}")
+
+
+def assert_both(name, gpu, ref):
+    assert_close(name, gpu, ref)
+    assert_exact(name, gpu, ref)
 
 
 def pair(w, options, oracle_mod):
@@ -87,18 +91,18 @@ def test_neighbor_search_bit_exact(workload, sort, oracle_mod):
         assert_exact(name, sim.download(name), orc.download(name))
 
 
-@pytest.mark.parametrize("arith", [fe.FE_ARITH_FAST, fe.FE_ARITH_REFERENCE])
-def test_stage_by_stage(workload, arith, oracle_mod):
-    """Every ComputeStep from identical inputs, in both arithmetics."""
+def test_stage_by_stage(workload, oracle_mod):
+    """Every ComputeStep from identical inputs (the stated bar is 1e-5 relative; the HIP
+    kernels evaluate the reference's operation sequence, so they also agree bit for bit)."""
     rng = np.random.default_rng(7)
-    sim, orc = pair(workload, opts(arith=arith), oracle_mod)
+    sim, orc = pair(workload, opts(), oracle_mod)
     n = workload.n
     sim.neighbor_search()
     orc.neighbor_search()
     sim.density_alpha()
     orc.density_alpha()
-    assert_close("densities", sim.download("densities"), orc.download("densities"))
-    assert_close("factors", sim.download("factors"), orc.download("factors"))
+    assert_both("densities", sim.download("densities"), orc.download("densities"))
+    assert_both("factors", sim.download("factors"), orc.download("factors"))
     sync_fields(sim, orc, ["densities", "factors"])
 
     v = np.zeros((n, 4), F)
@@ -106,12 +110,12 @@ def test_stage_by_stage(workload, arith, oracle_mod):
     for o in (sim, orc):
         o.upload("velocity_b", v)
         o.viscosity()
-    assert_close("viscosity velocity_a", sim.download("velocity_a"), orc.download("velocity_a"))
+    assert_both("viscosity velocity_a", sim.download("velocity_a"), orc.download("velocity_a"))
     sync_fields(sim, orc, ["velocity_a"])
 
     for o in (sim, orc):
         o.density_source()
-    assert_close("density source_terms", sim.download("source_terms"), orc.download("source_terms"))
+    assert_both("density source_terms", sim.download("source_terms"), orc.download("source_terms"))
     assert_exact("pressures zeroed", sim.download("pressures"), orc.download("pressures"))
     sync_fields(sim, orc, ["source_terms"])
 
@@ -119,18 +123,18 @@ def test_stage_by_stage(workload, arith, oracle_mod):
     for o in (sim, orc):
         o.upload("pressures", p)
         o.pressure_force()
-    assert_close("pressure_accelerations", sim.download("pressure_accelerations"),
+    assert_both("pressure_accelerations", sim.download("pressure_accelerations"),
                  orc.download("pressure_accelerations"))
     sync_fields(sim, orc, ["pressure_accelerations"])
     for o in (sim, orc):
         o.pressure_update()
-    assert_close("pressure_update pressures", sim.download("pressures"), orc.download("pressures"))
+    assert_both("pressure_update pressures", sim.download("pressures"), orc.download("pressures"))
     sync_fields(sim, orc, ["pressures"])
 
     for o in (sim, orc):
         o.pressure_integration()
-    assert_close("integrated position_a", sim.download("position_a"), orc.download("position_a"))
-    assert_close("integrated velocity_a", sim.download("velocity_a"), orc.download("velocity_a"))
+    assert_both("integrated position_a", sim.download("position_a"), orc.download("position_a"))
+    assert_both("integrated velocity_a", sim.download("velocity_a"), orc.download("velocity_a"))
     sync_fields(sim, orc, ["position_a", "velocity_a"])
 
     for o in (sim, orc):
@@ -142,12 +146,12 @@ def test_stage_by_stage(workload, arith, oracle_mod):
     sync_fields(sim, orc, ["densities", "factors"])
     for o in (sim, orc):
         o.divergence_source()
-    assert_close("divergence source_terms", sim.download("source_terms"), orc.download("source_terms"))
+    assert_both("divergence source_terms", sim.download("source_terms"), orc.download("source_terms"))
     sync_fields(sim, orc, ["source_terms"])
     for o in (sim, orc):
         o.divergence_integration()
-    assert_close("divergence velocity_a", sim.download("velocity_a"), orc.download("velocity_a"))
-    assert_close("colors", sim.download("colors"), orc.download("colors"))
+    assert_both("divergence velocity_a", sim.download("velocity_a"), orc.download("velocity_a"))
+    assert_both("colors", sim.download("colors"), orc.download("colors"))
     sync_fields(sim, orc, ["velocity_a"])
     for o in (sim, orc):
         o.stats()
@@ -161,7 +165,7 @@ FIELDS_STEP = ("position_a", "velocity_a", "position_b", "velocity_b", "densitie
 @pytest.mark.parametrize("sort", [fe.FE_SORT_RADIX, fe.FE_SORT_BITONIC])
 def test_frames_match_oracle(workload, sort, oracle_mod):
     """Whole frames in the reference arithmetic reproduce the oracle bit for bit."""
-    sim, orc = pair(workload, opts(sort=sort, arith=fe.FE_ARITH_REFERENCE), oracle_mod)
+    sim, orc = pair(workload, opts(sort=sort), oracle_mod)
     for frame in range(3):
         sim.step()
         orc.step()
@@ -178,16 +182,11 @@ def rms(x):
 
 
 @pytest.mark.parametrize("scene_name", ["rest_density_4800", "dam_break_32k"])
-def test_fast_density_phase_as_accurate_as_fp32_reference(scene_name, oracle_mod):
-    """The fast arithmetic's error against an FP64 evaluation of the same frame is no larger
-    than the reference FP32 arithmetic's. The density phase (neighbour search through
-    integration, 4 Jacobi sweeps) runs from one identical FP32 state in three evaluators:
-    the GPU fast path, the FP32 oracle and the FP64 oracle. The source term
-    (rho0 - rho) / dt cancels, so any FP32 evaluation order moves the result by ~1e-4
-    relative; the bound is on the error relative to FP64, not on the distance to one
-    particular FP32 order."""
+def test_fp64_error_report(scene_name, oracle_mod):
+    """FP64 re-evaluation of the density phase (neighbour search through integration, four
+    Jacobi sweeps) from one FP32 state: the GPU's per-field error against FP64 is stated and
+    equals the FP32 reference arithmetic's (the two agree bit for bit)."""
     w = SCENES[scene_name]()
-    # a moving state: three reference frames first
     warm = oracle_mod.Oracle(w.params, w.positions)
     for _ in range(3):
         warm.step()
@@ -209,48 +208,64 @@ def test_fast_density_phase_as_accurate_as_fp32_reference(scene_name, oracle_mod
     for name in ("densities", "factors", "source_terms", "pressures", "pressure_accelerations", "velocity_a",
                  "position_a"):
         ref = o64.download(name, as_double=True)
-        e_gpu = rms(sim.download(name) - ref)
-        e_32 = rms(o32.download(name) - ref)
+        g = sim.download(name)
+        assert_exact(name, g, o32.download(name))
         scale = rms(ref)
-        if scale == 0.0:          # e.g. all pressures clamped to 0 in an under-dense block
-            assert e_gpu == 0.0, name
-            continue
-        print(f"[fp64 check] {scene_name} {name}: rms err fast {e_gpu / scale:.3e}, fp32 ref {e_32 / scale:.3e}")
-        assert e_gpu <= 4.0 * e_32 + 1e-7 * scale, name
+        err = np.abs(g.astype(np.float64) - ref)
+        rel = err.max() / scale if scale else 0.0
+        print(f"[fp64] {scene_name} {name}: max |fp32 - fp64| / rms = {rel:.3e}, rms err / rms = "
+              f"{(rms(err) / scale if scale else 0.0):.3e}")
+        assert np.isfinite(g).all()
+
+
+def test_list_fallbacks_match_oracle(oracle_mod):
+    """Particles with more than 64 gradient neighbours, and a neighbour list made stale by
+    an upload of position_b, both take the bucket walk and still match the oracle."""
+    rng = np.random.default_rng(3)
+    w = scene.dam_break_32k(seed=4)
+    pos = w.positions.copy()
+    # a clump of 300 particles inside one half-smoothing-length cube: ~299 pairs each
+    c0 = np.array([0.5, 0.3, 0.5], np.float32)
+    pos[:300] = c0 + rng.uniform(0, 0.02, (300, 3)).astype(np.float32)
+    sim, orc = pair(scene.Workload("clump", pos, w.params), opts(), oracle_mod)
+    for o in (sim, orc):
+        o.neighbor_search()
+        o.density_alpha()
+        o.viscosity()
+        o.density_source()
+        o.pressure_force()
+        o.pressure_update()
+    for name in ("densities", "factors", "velocity_a", "source_terms", "pressure_accelerations", "pressures"):
+        assert_exact("clump " + name, sim.download(name), orc.download(name))
+    # stale list: move position_b after density_alpha; the reference gathers use the moved
+    # positions, so must the HIP path
+    pb = orc.download("position_b")
+    pb[:, :3] += rng.normal(0, 0.002, (pb.shape[0], 3)).astype(np.float32)
+    for o in (sim, orc):
+        o.upload("position_b", pb)
+        o.viscosity()
+        o.pressure_force()
+    for name in ("velocity_a", "pressure_accelerations"):
+        assert_exact("stale-list " + name, sim.download(name), orc.download(name))
 
 
 def test_converge_mode_iterations_match(oracle_mod):
     w = scene.dam_break_32k(seed=2)
-    o = opts(mode=fe.FE_SOLVER_CONVERGE, maxit=100, arith=fe.FE_ARITH_REFERENCE)
+    o = opts(mode=fe.FE_SOLVER_CONVERGE, maxit=100)
     sim, orc = pair(w, o, oracle_mod)
     for frame in range(3):
         sim.step()
         orc.step()
         assert sim.last_iterations() == orc.last_iterations(), frame
         for name in ("position_a", "velocity_a", "pressures"):
-            assert_close(f"converge frame {frame} {name}", sim.download(name), orc.download(name))
+            assert_exact(f"converge frame {frame} {name}", sim.download(name), orc.download(name))
     d, v, frames = sim.iteration_totals()
     assert frames == 3 and d >= 3 and v >= 3
 
 
-def test_converge_mode_fast_arithmetic_sweeps(oracle_mod):
-    """The fast path reaches the same thresholds in the same number of sweeps (+-1: the
-    residual is an FP32 sum compared against a threshold)."""
-    w = scene.dam_break_32k(seed=2)
-    sim, orc = pair(w, opts(mode=fe.FE_SOLVER_CONVERGE, maxit=100), oracle_mod)
-    for frame in range(3):
-        sim.step()
-        orc.step()
-        (gd, gv), (od, ov) = sim.last_iterations(), orc.last_iterations()
-        assert abs(gd - od) <= 1 and abs(gv - ov) <= 1, (frame, (gd, gv), (od, ov))
-        # keep the two runs on one trajectory
-        for k in ("position_a", "velocity_a", "pressures"):
-            sim.upload(k, orc.download(k))
-
-
 def test_host_flags_cfl_and_early_exit(oracle_mod):
     w = scene.rest_density_scene()
-    o = opts(use_cfl=1, use_solver_error_threshold=1, arith=fe.FE_ARITH_REFERENCE)
+    o = opts(use_cfl=1, use_solver_error_threshold=1)
     sim, orc = pair(w, o, oracle_mod)
     for frame in range(4):
         sim.step(w.params.dt)
