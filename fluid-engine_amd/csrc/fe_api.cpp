@@ -264,6 +264,10 @@ fe_status neighbor_search(fe_ctx* c) {
     FE_HIP(fe::launch_morton_keys(c), "morton keys");
     FE_HIP(fe::radix_sort_pairs(c, c->mkeys, c->sched, c->n, c->morton_bits, "schedule"), "schedule sort");
     FE_HIP(fe::launch_schedule_mirrors(c), "schedule mirrors");
+    c->ns_fresh = true;
+    c->da_fresh = false;
+    c->ns_sort = c->options.sort_algorithm;
+    c->ns_h = c->params.smoothing_radius;
     return FE_OK;
 }
 
@@ -284,8 +288,17 @@ fe_status step(fe_ctx* c, float max_dt) {
         frame_control(c->prev_stats, c->n, &c->params, &c->options, &dit, &vit);
     }
     fe_status s;
-    if ((s = neighbor_search(c)) != FE_OK) return s;
-    FE_HIP(fe::launch_density_alpha(c), "density_alpha");
+    // Frame-start neighbour search + density_alpha (src/renderer/mod.rs:279-286). When the
+    // particles have not moved since the last search (one substep per frame: the divergence
+    // phase leaves position_a alone), hash, sort, offsets, position_b, the densities and the
+    // factors would come out bit-identical, so only velocity_b is gathered again.
+    if (c->ns_fresh && c->da_fresh && c->ns_sort == c->options.sort_algorithm &&
+        c->ns_h == c->params.smoothing_radius) {
+        FE_HIP(fe::launch_reorder_velocity(c), "reorder velocity");
+    } else {
+        if ((s = neighbor_search(c)) != FE_OK) return s;
+        FE_HIP(fe::launch_density_alpha(c), "density_alpha");
+    }
     float t = 0.0f;
     while (t < max_dt) {
         FE_HIP(fe::launch_viscosity(c), "viscosity");
@@ -408,7 +421,7 @@ fe_status fe_create(const fe_params* params, const float* xyz, uint32_t n, int d
     ALLOC(c->resid, sizeof(unsigned long long) * 2 * fe::kMaxSolveIters);
     ALLOC(c->partial, sizeof(unsigned long long) * fe::gather_partials(n));
     ALLOC(c->stats_partial, sizeof(uint32_t) * 3 * fe::kStatsBlocks);
-    ALLOC(c->nbr, sizeof(uint32_t) * fe::kMaxNbr * (size_t)n);
+    ALLOC(c->nbr, sizeof(uint32_t) * fe::kMaxNbr * (((size_t)n + 15) / 16 * 16));   // whole 16-item tiles
     ALLOC(c->nbr_count, sizeof(uint32_t) * (size_t)n);
     for (int k = 0; k < 2; k++) {
         ALLOC(c->sched[k], sizeof(uint32_t) * (size_t)n);
@@ -517,6 +530,8 @@ fe_status fe_upload(fe_ctx* c, int field, const void* src, size_t bytes) {
     DeviceGuard g(c->device);
     if (field == FE_FIELD_POSITION_B || field == FE_FIELD_GRID_ENTRIES || field == FE_FIELD_GRID_START)
         c->list_valid = false;
+    c->ns_fresh = false;
+    c->da_fresh = false;
     // schedule-ordered mirrors of the uploaded field are stale now
     switch (field) {
         case FE_FIELD_POSITION_B: c->mirrors &= ~fe::kMirrorPos; break;
@@ -598,6 +613,8 @@ FE_STAGE(fe_sort, {
     else
         FE_HIP(fe::radix_sort(c, c->T, 32), "radix sort");
     c->ends_dirty = true;
+    c->ns_fresh = false;
+    c->da_fresh = false;
     c->list_valid = false;
 })
 FE_STAGE(fe_density_alpha, {

@@ -1,14 +1,14 @@
 // fe_gather.hip — the neighbour gathers of the DFSPH solver on CDNA4, in the reference's
 // own arithmetic.
 //
-// density_alpha (density_and_alpha.comp:14-72) is the one stage that walks the 27 buckets
-// around each particle (z outer, x inner, each bucket in sorted order, a bucket visited
-// twice when two neighbour cells share a hash slot). While it does, it records every pair
-// that contributes a gradient term (r^2 <= h^2 and r > 1e-6) in that visit order. Every
-// later gather of the same neighbour structure — viscosity, both source terms and each
-// pressure_force / pressure_update sweep — visits exactly those pairs in exactly that
-// order: a pair the reference skips (r^2 > h^2, or r <= 1e-6 where only the density uses
-// it) contributes nothing there either. With IEEE division and square root and the GLSL
+// Every gather of the reference walks the 27 buckets around its particle (z outer, x
+// inner, each bucket in sorted order, a bucket visited twice when two neighbour cells
+// share a hash slot; density_and_alpha.comp:31-45 and the same loop elsewhere) and uses the
+// candidates with r^2 <= h^2. After each neighbour search that walk is done once
+// (build_list) and every accepted pair is recorded in visit order; density_alpha,
+// viscosity, both source terms and each pressure_force / pressure_update sweep then
+// visit exactly those pairs in exactly that order — a rejected candidate contributes
+// nothing in the reference either. With IEEE division and square root and the GLSL
 // operation order, each particle's sums are the reference's sums, bit for bit.
 //
 // Memory layout for the gathers. The reference's arrays stay in hash-sorted order (they
@@ -18,8 +18,9 @@
 // particle sched[t]) and the data a gather reads from its neighbours is kept in
 // schedule-ordered mirrors (pos_m, vb_m, va_m, dens_m, prho_m = p / rho^2, pacc_m) that the
 // producing stage writes next to its hash-ordered output. The list holds schedule
-// positions, slot-major (nbr[k * N + t]), so a wave reads slot k of 64 work-items as 256
-// contiguous bytes and each lane keeps four independent neighbour loads in flight.
+// positions in tiles of 16 work-items x 4 slots, so 16 lanes reading four entries each
+// read 256 contiguous bytes, a cell group's entries fill whole lines while the group is
+// built, and each lane keeps four independent neighbour loads in flight.
 // Blocks are remapped so that each XCD (one L2 each) walks a contiguous range of the
 // Morton order. A particle with more than kMaxNbr pairs, or a stale list, falls back to
 // the bucket walk, which produces the same pairs in the same order.
@@ -122,6 +123,19 @@ __global__ __launch_bounds__(kBlock) void k_schedule_mirrors(const uint32_t* __r
     vb_m[t] = vel_b[i];
 }
 
+// reorder.comp:44 for the velocities alone, written to velocity_b and its mirror.
+__global__ __launch_bounds__(kBlock) void k_reorder_velocity(const uint32_t* __restrict__ sched,
+                                                             const uint32_t* __restrict__ vals, uint32_t n,
+                                                             const float4* __restrict__ vel_a,
+                                                             float4* __restrict__ vel_b, float4* __restrict__ vb_m) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = sched[t];
+    const float4 v = vel_a[vals[i]];
+    vel_b[i] = v;
+    vb_m[t] = v;
+}
+
 // Every mirror from the hash-ordered fields (after uploads or out-of-sequence calls).
 __global__ __launch_bounds__(kBlock) void k_refresh_mirrors(const uint32_t* __restrict__ sched, uint32_t n,
                                                             const float4* __restrict__ pos_b,
@@ -146,50 +160,196 @@ __global__ __launch_bounds__(kBlock) void k_refresh_mirrors(const uint32_t* __re
     pacc_m[t] = pacc[i];
 }
 
-// ---- density_alpha: bucket walk + list build ---------------------------------------------
+// ---- neighbour list build -----------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_density_alpha(Sph s, const uint2* __restrict__ range,
-                                                          const float4* __restrict__ pos,
-                                                          const uint32_t* __restrict__ sched,
-                                                          const uint32_t* __restrict__ inv, float* __restrict__ dens,
-                                                          float* __restrict__ fac, float* __restrict__ dens_m,
-                                                          uint32_t* __restrict__ nbr, uint32_t* __restrict__ cnt) {
-    const uint32_t t = work_item();
-    if (t >= s.n) return;
-    const uint32_t i = sched[t];
-    const float4 pi = pos[i];
-    float density = 0.0f, sum_grad_sq = 0.0f;
-    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    uint32_t m = 0;
-    visit27(range, pos, s, pi.x, pi.y, pi.z, [&](uint32_t j, const float4& pj) {
-        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 > s.hh) return;
-        const float r = sqrtf(r2);
-        const float w = exact_w(r, s);
-        density += s.mass * w;
-        if (r > 1e-6f) {
-            if (m < kMaxNbr) nbr[(size_t)m * s.n + t] = inv[j];
-            m++;
-            const float g = exact_grad(r, s);
-            const float mx = s.mass * (g * rx), my = s.mass * (g * ry), mz = s.mass * (g * rz);
-            gx += mx;
-            gy += my;
-            gz += mz;
-            sum_grad_sq += mx * mx + my * my + mz * mz;
+// Entry `slot` of work-item t in the tiled list: tiles of 16 work-items x 4 slots (256 B),
+// so 16 lanes reading 4 slots each touch 256 contiguous bytes.
+__device__ __forceinline__ size_t list_at(uint32_t t, uint32_t slot) {
+    return ((((size_t)(t >> 4) * (kMaxNbr / 4) + (slot >> 2)) << 6) + ((t & 15u) << 2) + (slot & 3u));
+}
+
+constexpr int kWaves = kBlock / 64;
+constexpr uint32_t kCand = 256;   // candidates staged per pass per wave
+
+struct BuildStage {
+    float4 pos[kCand];   // candidate positions (hash order position_b)
+    uint32_t idx[kCand]; // hash index, then schedule position of each candidate
+};
+
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x - v;
+}
+
+// One cell group (a run of consecutive work-items whose particles share a cell): lanes
+// 0..26 fetch the 27 bucket ranges in visit order, the wave stages every candidate's
+// position and schedule position in LDS, then S = 64 / (group size) lanes per particle
+// test S consecutive candidates per round; a ballot numbers the accepted ones so each
+// particle's list comes out in candidate (= reference visit) order.
+__device__ void build_group(const Sph& s, const uint2* __restrict__ range, const float4* __restrict__ pos_b,
+                            const float4* __restrict__ pos_m, const uint32_t* __restrict__ inv,
+                            uint32_t* __restrict__ nbr, uint32_t* __restrict__ cnt, BuildStage& st, uint32_t gs,
+                            uint32_t ge, int3 gc) {
+    const int lane = threadIdx.x & 63;
+    uint32_t bst = 0, bcnt = 0;
+    if (lane < 27) {
+        const int dz = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dx = lane % 3 - 1;
+        const uint2 r = range[cell_hash(gc.x + dx, gc.y + dy, gc.z + dz, s.mask)];
+        if (r.x != kSentinel) {
+            bst = r.x;
+            bcnt = r.y - r.x;
         }
-    });
-    cnt[t] = m;
-    dens[i] = density;
-    dens_m[t] = density;
-    const float alpha = sum_grad_sq + (gx * gx + gy * gy + gz * gz);
-    fac[i] = alpha > 1e-6f ? alpha : 0.0f;
+    }
+    const uint32_t boff = wave_excl_scan(bcnt);
+    const uint32_t C = __shfl(boff + bcnt, 26, 64);
+    const uint32_t np = ge - gs;
+    uint32_t S = 64;
+    while (S > 1 && S * np > 64u) S >>= 1;
+    const uint32_t per_chunk = 64u / S;
+    const uint64_t segmask = S == 64 ? ~0ull : ((1ull << S) - 1ull);
+    const uint32_t sub = (uint32_t)lane & (S - 1u);
+    const uint32_t seg_shift = (uint32_t)lane & ~(S - 1u);
+    uint32_t staged = 0xFFFFFFFFu;
+
+    for (uint32_t pb = gs; pb < ge; pb += per_chunk) {
+        const uint32_t tp = pb + (uint32_t)lane / S;
+        const bool act = tp < ge;
+        const float4 pi = pos_m[act ? tp : gs];
+        uint32_t m = 0;
+        for (uint32_t cb = 0; cb < C; cb += kCand) {
+            const uint32_t cn = (C - cb) < kCand ? (C - cb) : kCand;
+            if (staged != cb) {
+                wave_sync();
+                if (bcnt) {
+                    const uint32_t lo = boff > cb ? boff : cb;
+                    const uint32_t hi = (boff + bcnt) < (cb + cn) ? (boff + bcnt) : (cb + cn);
+                    for (uint32_t q = lo; q < hi; q++) st.idx[q - cb] = bst + (q - boff);
+                }
+                wave_sync();
+                for (uint32_t q = lane; q < cn; q += 64) {
+                    const uint32_t j = st.idx[q];
+                    st.pos[q] = pos_b[j];
+                    st.idx[q] = inv[j];
+                }
+                wave_sync();
+                staged = cb;
+            }
+            for (uint32_t r = 0; r < cn; r += S) {
+                const uint32_t q = r + sub;
+                bool ok = false;
+                if (act && q < cn) {
+                    const float4 pj = st.pos[q];
+                    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                    ok = !(rx * rx + ry * ry + rz * rz > s.hh);
+                }
+                const uint64_t seg = (__ballot(ok) >> seg_shift) & segmask;
+                if (ok) {
+                    const uint32_t slot = m + (uint32_t)__popcll(seg & ((1ull << sub) - 1ull));
+                    if (slot < kMaxNbr) nbr[list_at(tp, slot)] = st.idx[q];
+                }
+                m += (uint32_t)__popcll(seg);
+            }
+        }
+        if (act && sub == 0) cnt[tp] = m;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_build_list(Sph s, const uint2* __restrict__ range,
+                                                       const float4* __restrict__ pos_b,
+                                                       const float4* __restrict__ pos_m,
+                                                       const uint32_t* __restrict__ inv, uint32_t* __restrict__ nbr,
+                                                       uint32_t* __restrict__ cnt) {
+    __shared__ BuildStage stage[kWaves];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t base = (xcd_block() * kWaves + w) * 64u;
+    if (base >= s.n) return;
+    const uint32_t t = base + lane;
+    const bool valid = t < s.n;
+    const float4 p = valid ? pos_m[t] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int3 c = cell_of(p.x, p.y, p.z, s.inv_h);
+    int3 cp = make_int3(__shfl_up(c.x, 1, 64), __shfl_up(c.y, 1, 64), __shfl_up(c.z, 1, 64));
+    if (lane == 0 && base > 0) {
+        const float4 q = pos_m[base - 1];
+        cp = cell_of(q.x, q.y, q.z, s.inv_h);
+    }
+    const bool start = valid && (t == 0 || c.x != cp.x || c.y != cp.y || c.z != cp.z);
+    uint64_t starts = __ballot(start);
+    while (starts) {
+        const int sl = __builtin_ctzll(starts);
+        starts &= starts - 1;
+        const uint32_t gs = base + (uint32_t)sl;
+        const int3 gc = make_int3(__shfl(c.x, sl, 64), __shfl(c.y, sl, 64), __shfl(c.z, sl, 64));
+        uint32_t ge;
+        if (starts) {
+            ge = base + (uint32_t)__builtin_ctzll(starts);
+        } else {
+            // the group runs to the end of the window, and possibly beyond it
+            ge = base + 64u < s.n ? base + 64u : s.n;
+            while (ge < s.n) {
+                const uint32_t j = ge + lane;
+                const bool vj = j < s.n;
+                const float4 q = vj ? pos_m[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const int3 cj = cell_of(q.x, q.y, q.z, s.inv_h);
+                const uint64_t diff = __ballot(!vj || cj.x != gc.x || cj.y != gc.y || cj.z != gc.z);
+                if (diff) {
+                    ge += (uint32_t)__builtin_ctzll(diff);
+                    break;
+                }
+                ge += 64u;
+            }
+        }
+        build_group(s, range, pos_b, pos_m, inv, nbr, cnt, stage[w], gs, ge, gc);
+    }
 }
 
 // ---- list consumers: one policy per reference shader ------------------------------------
 // pair() repeats the reference's own tests (r^2 > h^2 rejects, gradient terms need
 // r > 1e-6), so the same code serves list entries (which pass them) and the bucket walk.
 // load(jt) reads neighbour data by schedule position jt.
+
+// density_and_alpha.comp:14-72 over every pair with r^2 <= h^2 (the list includes the
+// particle itself and any r <= 1e-6 pair, which add to the density only).
+struct DensityAlpha {
+    float* dens;
+    float* fac;
+    float* dens_m;
+    struct Self {};
+    struct Cand {};
+    struct Acc { float rho, sgs, gx, gy, gz; };
+    __device__ Self self(uint32_t, uint32_t) const { return {}; }
+    __device__ Cand load(uint32_t) const { return {}; }
+    __device__ Acc zero() const { return {0.f, 0.f, 0.f, 0.f, 0.f}; }
+    __device__ void pair(Acc& a, const Self&, const Sph& s, float rx, float ry, float rz, const Cand&) const {
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 > s.hh) return;
+        const float r = sqrtf(r2);
+        const float w = exact_w(r, s);
+        a.rho += s.mass * w;
+        if (r > 1e-6f) {
+            const float g = exact_grad(r, s);
+            const float mx = s.mass * (g * rx), my = s.mass * (g * ry), mz = s.mass * (g * rz);
+            a.gx += mx;
+            a.gy += my;
+            a.gz += mz;
+            a.sgs += mx * mx + my * my + mz * mz;
+        }
+    }
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&) const {
+        dens[i] = a.rho;
+        dens_m[t] = a.rho;
+        const float alpha = a.sgs + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
+        fac[i] = alpha > 1e-6f ? alpha : 0.0f;
+        return 0;
+    }
+};
 
 // viscosity.comp:17-66 (velocity_b in, velocity_a out with w = 0, gravity fused)
 struct Viscosity {
@@ -417,10 +577,11 @@ __global__ __launch_bounds__(kBlock) void k_list_gather(P pol, Sph s, const uint
         typename P::Acc acc = pol.zero();
         const uint32_t m = list_ok ? cnt[t] : 0xFFFFFFFFu;
         if (m <= kMaxNbr) {
+            const uint4* tile = reinterpret_cast<const uint4*>(nbr) + (size_t)(t >> 4) * (kMaxNbr / 4) * 16 + (t & 15u);
             for (uint32_t k = 0; k < m; k += 4) {
-                uint32_t jt[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) jt[u] = nbr[(size_t)((k + u < m) ? k + u : k) * s.n + t];
+                const uint4 q4 = tile[(k >> 2) * 16];
+                const uint32_t jt[4] = {q4.x, k + 1 < m ? q4.y : q4.x, k + 2 < m ? q4.z : q4.x,
+                                        k + 3 < m ? q4.w : q4.x};
                 float4 pj[4];
                 typename P::Cand cj[4];
 #pragma unroll
@@ -502,15 +663,30 @@ hipError_t launch_schedule_mirrors(fe_ctx* c) {
     return hipGetLastError();
 }
 
-hipError_t launch_density_alpha(fe_ctx* c) {
-    hipError_t e = need(c, kMirrorPos);   // the inverse schedule
-    if (e != hipSuccess) return e;
-    Launch L(c, "density_alpha");
-    hipLaunchKernelGGL(k_density_alpha, grid_for(c->n), dim3(kBlock), 0, c->stream, sph_of(c), c->range, c->pos_b,
-                       c->sched[0], c->inv, c->dens, c->fac, c->dens_m, c->nbr, c->nbr_count);
-    c->list_valid = true;
-    c->mirrors = (c->mirrors | kMirrorDens) & ~kMirrorPrho;   // p / rho^2 used the old densities
+hipError_t launch_reorder_velocity(fe_ctx* c) {
+    Launch L(c, "reorder_velocity");
+    hipLaunchKernelGGL(k_reorder_velocity, grid_for(c->n), dim3(kBlock), 0, c->stream, c->sched[0], c->vals[0], c->n,
+                       c->vel_a, c->vel_b, c->vb_m);
+    c->mirrors |= kMirrorVelB;
     return hipGetLastError();
+}
+
+hipError_t launch_density_alpha(fe_ctx* c) {
+    hipError_t e = need(c, kMirrorPos);   // positions by schedule position + inverse schedule
+    if (e != hipSuccess) return e;
+    {
+        Launch L(c, "build_list");
+        hipLaunchKernelGGL(k_build_list, grid_for(c->n), dim3(kBlock), 0, c->stream, sph_of(c), c->range, c->pos_b,
+                           c->pos_m, c->inv, c->nbr, c->nbr_count);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    c->list_valid = true;
+    DensityAlpha pol{c->dens, c->fac, c->dens_m};
+    e = run_list(c, "density_alpha", pol);
+    c->mirrors = (c->mirrors | kMirrorDens) & ~kMirrorPrho;   // p / rho^2 used the old densities
+    c->da_fresh = c->ns_fresh;
+    return e;
 }
 
 hipError_t launch_viscosity(fe_ctx* c) {

@@ -91,6 +91,13 @@ struct fe_ctx {
     float* prho_m = nullptr;     // pressure / density^2
     float4* pacc_m = nullptr;    // pressure_accelerations
     uint32_t mirrors = 0;        // kMirror* bits that are current
+    // The neighbour structure and densities still describe position_a (nothing moved the
+    // particles since the last neighbour search + density_alpha): the next frame-start
+    // search would reproduce them bit for bit, so only velocity_b is re-gathered.
+    bool ns_fresh = false;
+    bool da_fresh = false;
+    uint32_t ns_sort = 0;
+    float ns_h = 0.0f;
 
     fe::SortScratch sort;
 
@@ -149,6 +156,8 @@ void sort_scratch_free(SortScratch* s);
 uint32_t gather_partials(uint32_t n);
 // Inverse schedule and the position / velocity_b mirrors after the schedule sort.
 hipError_t launch_schedule_mirrors(fe_ctx* c);
+// velocity_b = velocity_a gathered through the current sort (reorder.comp's velocity half).
+hipError_t launch_reorder_velocity(fe_ctx* c);
 
 // ---- launchers (fe_sph.hip) ----
 hipError_t launch_hash(fe_ctx* c);

@@ -349,6 +349,8 @@ hipError_t launch_rebuild_ends(fe_ctx* c) {
 
 hipError_t launch_pressure_integration(fe_ctx* c) {
     c->mirrors &= ~kMirrorVelA;
+    c->ns_fresh = false;   // position_a moves
+    c->da_fresh = false;
     Launch L(c, "pressure_integration");
     const fe_params& p = c->params;
     hipLaunchKernelGGL(k_pressure_integration, grid_for(c->n), dim3(kBlock), 0, c->stream, c->n, c->pos_b, c->pacc,
