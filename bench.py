@@ -53,11 +53,41 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_probe():
+    """A plain device-to-device copy of 1 GiB whose source was pushed out of the 256 MB
+    MALL by a later 1 GiB write: the HBM bytes per dispatch are known exactly (1 GiB read,
+    1 GiB written), which calibrates the traffic counters."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    z = torch.full((n,), 2.0, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "check": float(y[-1]) + float(z[0])}), flush=True)
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "traffic_dam_break_1m.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f).get("kernels", {}).get(kernel)
+    return None if t is None else t.get("hbm_bytes_per_launch")
 
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py: this build runs one scene on one GPU; --gpus {args.gpus} / WORLD_SIZE {world} "
@@ -90,8 +120,19 @@ def main():
     d_it, v_it, frames = sim.iteration_totals()
     print(f"[bench] timed {args.steps} steps: {ms:.2f} ms (wall {wall * 1e3:.2f} ms)", file=sys.stderr, flush=True)
 
-    # per-kernel durations: every launch of a few more steps bracketed by HIP events on
-    # the context's stream
+    # Per-kernel durations: every launch of a few more steps bracketed by HIP events on the
+    # context's stream. These steps run the timed run's mean sweep counts as fixed sweeps, so
+    # every pressure_force / pressure_update launch does a full sweep (in converge mode the
+    # launches queued past convergence return at once and would dilute the average).
+    d_avg = max(1, int(round(d_it / max(frames, 1))))
+    v_avg = max(1, int(round(v_it / max(frames, 1))))
+    prof_opts = fe.FeOptions.from_buffer_copy(opts)
+    prof_opts.solver_mode = fe.FE_SOLVER_FIXED
+    prof_params = fe.FeParams.from_buffer_copy(sim.params)
+    prof_params.density_solver_iterations = d_avg
+    prof_params.divergence_solver_iterations = v_avg
+    sim.set_options(prof_opts)
+    sim.set_params(prof_params)
     sim.profile(True)
     sim.profile_reset()
     sim.run(args.profile_steps, dt)
@@ -137,8 +178,9 @@ def main():
         "particle_updates_per_s": round(steps_per_s * w.n, 1),
         "jacobi_sweeps_per_step": {"density": round(d_it / max(frames, 1), 3),
                                    "divergence": round(v_it / max(frames, 1), 3)},
+        "profile_sweeps": {"density": d_avg, "divergence": v_avg},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(dom),
                      "algo_bytes_per_particle": dom_bytes, "avg_launch_us": kernels[dom]["avg_us"]},
         "cpu_baseline": cpu,
     }
