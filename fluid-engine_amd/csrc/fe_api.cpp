@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "fe_internal.h"
+#include "fe_abi.h"
 
 static_assert(sizeof(fe_params) == 112, "fe_params must match SimulationParams (112 bytes)");
 static_assert(offsetof(fe_params, dt) == 24, "dt offset");
@@ -20,42 +20,19 @@ static_assert(offsetof(fe_params, box_max) == 80, "box_max offset");
 static_assert(offsetof(fe_params, grid_res) == 96, "grid_res offset");
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-fe_status fail(fe_status code, const std::string& msg) {
+fe_status fe::fail(fe_status code, const std::string& msg) {
     g_last_error = msg;
     return code;
 }
 
-fe_status hip_fail(hipError_t e, const char* what) {
-    return fail(e == hipErrorOutOfMemory ? FE_ERR_OUT_OF_MEMORY : FE_ERR_DEVICE,
-                std::string(what) + ": " + hipGetErrorString(e));
-}
+namespace {
 
-#define FE_HIP(call, what)                        \
-    do {                                          \
-        hipError_t e_ = (call);                   \
-        if (e_ != hipSuccess) return hip_fail(e_, what); \
-    } while (0)
-
-#define FE_CHECK_CTX(c)                                                        \
-    do {                                                                       \
-        if (!(c)) return fail(FE_ERR_INVALID_ARGUMENT, "null fe_ctx");         \
-    } while (0)
-
-// Every entry point that launches work makes the context's device current.
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        if (prev != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        int cur = -1;
-        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-    }
-};
+using fe::DeviceGuard;
+using fe::fail;
+using fe::hip_fail;
 
 bool field_info(const fe_ctx* c, int field, size_t* bytes, void** dev) {
     const size_t n = c->n;
@@ -88,6 +65,7 @@ void free_ctx(fe_ctx* c) {
                     (void*)c->prho_m, (void*)c->pacc_m})
         if (p) (void)hipFree(p);
     fe::sort_scratch_free(&c->sort);
+    fe::learn_free(c);
     if (c->host_stats) (void)hipHostFree(c->host_stats);
     if (c->host_resid) (void)hipHostFree(c->host_resid);
     for (auto& r : c->pending) {
@@ -162,13 +140,20 @@ void frame_control(const uint32_t stats[3], uint32_t n, fe_params* p, const fe_o
     *vit = (o->use_solver_error_threshold && avg_ve < vthr) ? 1u : p->divergence_solver_iterations;
 }
 
-// Jacobi loop (src/renderer/mod.rs:301-306 / :327-332). kind 0 = density, 1 = divergence.
+fe_status jacobi(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t* iterations);
+
+// Jacobi loop (src/renderer/mod.rs:301-306 / :327-332) between the warm-start hooks.
+// kind 0 = density, 1 = divergence.
 fe_status solve(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t* iterations) {
     FE_CHECK_CTX(c);
-    {
-        fe_status s = ensure_ends(c);
-        if (s != FE_OK) return s;
-    }
+    fe_status s = ensure_ends(c);
+    if (s == FE_OK) s = fe::learn_before_solve(c, kind);
+    if (s == FE_OK) s = jacobi(c, kind, fixed_iters, tol, iterations);
+    if (s == FE_OK) s = fe::learn_after_solve(c, kind);
+    return s;
+}
+
+fe_status jacobi(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t* iterations) {
     if (c->options.solver_mode != FE_SOLVER_CONVERGE) {
         for (uint32_t k = 0; k < fixed_iters; k++) {
             FE_HIP(fe::launch_pressure_force(c, nullptr, 0), "pressure_force");
@@ -497,8 +482,11 @@ fe_status fe_set_options(fe_ctx* c, const fe_options* o) {
         return fail(FE_ERR_INVALID_ARGUMENT, "unknown sort_algorithm");
     if (o->solver_mode != FE_SOLVER_FIXED && o->solver_mode != FE_SOLVER_CONVERGE)
         return fail(FE_ERR_INVALID_ARGUMENT, "unknown solver_mode");
-    if (o->warm_start != FE_WARM_START_REFERENCE)
-        return fail(FE_ERR_UNSUPPORTED, "warm_start mode not available in this build");
+    if (o->warm_start != FE_WARM_START_REFERENCE && o->warm_start != FE_WARM_START_LEARNED)
+        return fail(FE_ERR_INVALID_ARGUMENT, "unknown warm_start");
+    if (o->warm_start == FE_WARM_START_LEARNED && !fe::learn_ready(c, 0) && !fe::learn_ready(c, 1))
+        return fail(FE_ERR_UNSUPPORTED,
+                    "warm_start LEARNED needs a network: fe_learn_train or fe_learn_set_weights first");
     if (o->solver_mode == FE_SOLVER_CONVERGE && (o->max_iterations < 1 || o->max_iterations > fe::kMaxSolveIters))
         return fail(FE_ERR_INVALID_ARGUMENT, "max_iterations must be in [1, 1024]");
     c->options = *o;
@@ -532,6 +520,7 @@ fe_status fe_upload(fe_ctx* c, int field, const void* src, size_t bytes) {
         c->list_valid = false;
     c->ns_fresh = false;
     c->da_fresh = false;
+    fe::learn_invalidate(c);
     // schedule-ordered mirrors of the uploaded field are stale now
     switch (field) {
         case FE_FIELD_POSITION_B: c->mirrors &= ~fe::kMirrorPos; break;
@@ -616,6 +605,7 @@ FE_STAGE(fe_sort, {
     c->ns_fresh = false;
     c->da_fresh = false;
     c->list_valid = false;
+    fe::learn_invalidate(c);
 })
 FE_STAGE(fe_density_alpha, {
     fe_status s = ensure_ends(c);

@@ -78,4 +78,50 @@ __device__ __forceinline__ unsigned long long residual_q(float s_minus_ap, float
 // Hardware square root (1 ulp) for pair distances.
 __device__ __forceinline__ float pair_sqrt(float r2) { return __builtin_amdgcn_sqrtf(r2); }
 
+// common.glsl:38-54 kernel_w with IEEE q = r / h.
+__device__ __forceinline__ float exact_w(float r, const Sph& s) {
+    float q = r / s.h;
+    if (q >= 1.0f) return 0.0f;
+    if (q <= 0.5f) {
+        float q2 = q * q;
+        float q3 = q2 * q;
+        return s.k * (6.0f * q3 - 6.0f * q2 + 1.0f);
+    }
+    float omq = 1.0f - q;
+    float omq3 = omq * omq * omq;
+    return s.k * 2.0f * omq3;
+}
+
+// common.glsl:56-73 kernel_grad; the scalar that multiplies r_vec.
+__device__ __forceinline__ float exact_grad(float r, const Sph& s) {
+    float q = r / s.h;
+    if (q >= 1.0f || r < 1e-6f) return 0.0f;
+    float gf;
+    if (q <= 0.5f) {
+        gf = s.k * (18.0f * q * q - 12.0f * q);
+    } else {
+        float omq = 1.0f - q;
+        gf = s.k * -6.0f * omq * omq;
+    }
+    return gf / (s.h * r);
+}
+
+// p / rho^2 of a particle whose density is <= 1e-6: pressure_force skips such neighbours.
+// A signalling-NaN payload that no arithmetic result carries.
+constexpr uint32_t kNoRho = 0x7FA00001u;
+
+__device__ __forceinline__ float prho_of(float p, float rho) {
+    return rho > 1e-6f ? p / (rho * rho) : __uint_as_float(kNoRho);
+}
+
+// Schedule position of the k-th listed pair of work-item t: the list is stored in tiles of
+// 16 work-items x 4 slots (256 B), uint4 chunk (k / 4) of lane (t % 16).
+__device__ __forceinline__ const uint4* list_tile(const uint32_t* nbr, uint32_t t) {
+    return reinterpret_cast<const uint4*>(nbr) + (size_t)(t >> 4) * (kMaxNbr / 4) * 16 + (t & 15u);
+}
+__device__ __forceinline__ uint32_t list_pick(const uint4& q, uint32_t k) {
+    const uint32_t u = k & 3u;
+    return u == 0 ? q.x : u == 1 ? q.y : u == 2 ? q.z : q.w;
+}
+
 }  // namespace fe

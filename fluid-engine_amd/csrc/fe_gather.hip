@@ -29,14 +29,6 @@
 namespace fe {
 namespace {
 
-// p / rho^2 of a particle whose density is <= 1e-6: pressure_force skips such neighbours.
-// A signalling-NaN payload that no arithmetic result carries.
-constexpr uint32_t kNoRho = 0x7FA00001u;
-
-__device__ __forceinline__ float prho_of(float p, float rho) {
-    return rho > 1e-6f ? p / (rho * rho) : __uint_as_float(kNoRho);
-}
-
 // Workgroups are dealt round-robin over the 8 XCDs; give every XCD a contiguous range of
 // the Morton-ordered work.
 __device__ __forceinline__ uint32_t xcd_block() {
@@ -46,34 +38,6 @@ __device__ __forceinline__ uint32_t xcd_block() {
 }
 
 __device__ __forceinline__ uint32_t work_item() { return xcd_block() * kBlock + threadIdx.x; }
-
-// common.glsl:38-54 kernel_w with IEEE q = r / h.
-__device__ __forceinline__ float exact_w(float r, const Sph& s) {
-    float q = r / s.h;
-    if (q >= 1.0f) return 0.0f;
-    if (q <= 0.5f) {
-        float q2 = q * q;
-        float q3 = q2 * q;
-        return s.k * (6.0f * q3 - 6.0f * q2 + 1.0f);
-    }
-    float omq = 1.0f - q;
-    float omq3 = omq * omq * omq;
-    return s.k * 2.0f * omq3;
-}
-
-// common.glsl:56-73 kernel_grad; the scalar that multiplies r_vec.
-__device__ __forceinline__ float exact_grad(float r, const Sph& s) {
-    float q = r / s.h;
-    if (q >= 1.0f || r < 1e-6f) return 0.0f;
-    float gf;
-    if (q <= 0.5f) {
-        gf = s.k * (18.0f * q * q - 12.0f * q);
-    } else {
-        float omq = 1.0f - q;
-        gf = s.k * -6.0f * omq * omq;
-    }
-    return gf / (s.h * r);
-}
 
 // The 27-bucket walk in reference order over the hash-ordered positions. The nine ranges
 // of one z slab are loaded together and candidates are fetched four at a time.
@@ -577,7 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_list_gather(P pol, Sph s, const uint
         typename P::Acc acc = pol.zero();
         const uint32_t m = list_ok ? cnt[t] : 0xFFFFFFFFu;
         if (m <= kMaxNbr) {
-            const uint4* tile = reinterpret_cast<const uint4*>(nbr) + (size_t)(t >> 4) * (kMaxNbr / 4) * 16 + (t & 15u);
+            const uint4* tile = list_tile(nbr, t);
             for (uint32_t k = 0; k < m; k += 4) {
                 const uint4 q4 = tile[(k >> 2) * 16];
                 const uint32_t jt[4] = {q4.x, k + 1 < m ? q4.y : q4.x, k + 2 < m ? q4.z : q4.x,

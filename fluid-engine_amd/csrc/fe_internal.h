@@ -39,6 +39,49 @@ struct SortScratch {
     uint32_t max_tiles = 0;
 };
 
+// The learned warm start: two per-particle MLPs (density solve, divergence solve), each
+// kMlpIn -> kMlpHidden -> kMlpHidden -> kMlpHidden -> 1 with GELU.
+constexpr int kMlpIn = 12;
+constexpr int kMlpHidden = 64;
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + 2 * (kMlpHidden * kMlpHidden + kMlpHidden) +
+                           kMlpHidden + 1;
+constexpr int kMlpNorm = 2 * kMlpIn + 2;        // feature means, feature stds, target mean, target std
+constexpr uint32_t kMlpMaxBatch = 1u << 16;     // samples per training step
+constexpr uint32_t kMlpSamplesPerBlock = 32;    // one wave of the training kernel
+
+struct LearnState {
+    bool alloc = false;
+    // per network (0 density, 1 divergence): FP32 parameters, Adam moments, normalisation
+    float* params[2] = {nullptr, nullptr};
+    float* m1[2] = {nullptr, nullptr};
+    float* m2[2] = {nullptr, nullptr};
+    float* norm[2] = {nullptr, nullptr};
+    bool ready[2] = {false, false};
+    uint32_t adam_t[2] = {0, 0};
+    // per particle: features of the coming solve (schedule order), the previous pressure of
+    // each particle in the current hash order, and the last solve's final pressures (hash
+    // order of that solve)
+    float* feat = nullptr;
+    float* pprev = nullptr;
+    float* plast[2] = {nullptr, nullptr};
+    bool plast_valid[2] = {false, false};
+    // dataset (ring buffer per network) of (features, final pressure)
+    float* data_x[2] = {nullptr, nullptr};
+    float* data_t[2] = {nullptr, nullptr};
+    uint32_t capacity = 0;
+    uint32_t head[2] = {0, 0};
+    uint32_t size[2] = {0, 0};
+    uint32_t per_solve = 0;
+    uint32_t salt = 0;
+    // training scratch
+    float* train_x = nullptr;   // standardised copy of a dataset
+    float* train_t = nullptr;
+    float* grads = nullptr;     // per-block partial gradients
+    float* loss = nullptr;      // per-block partial losses
+    float* io = nullptr;        // staging of fe_learn_forward / fe_learn_gradient
+    size_t io_floats = 0;
+};
+
 struct Launch;
 
 }  // namespace fe
@@ -100,6 +143,7 @@ struct fe_ctx {
     float ns_h = 0.0f;
 
     fe::SortScratch sort;
+    fe::LearnState ws;
 
     // stats_buffer and the convergence residuals (one slot per sweep, two solves)
     uint32_t* stats = nullptr;
@@ -183,5 +227,22 @@ hipError_t launch_pack_entries(fe_ctx* c, uint32_t* out);
 hipError_t launch_unpack_entries(fe_ctx* c, const uint32_t* in);
 hipError_t launch_pack_start(fe_ctx* c, uint32_t* out);
 hipError_t launch_unpack_start(fe_ctx* c, const uint32_t* in);
+
+// ---- launchers (fe_learn.hip) ----
+// Previous pressure of every particle (ws.pprev) from ws.plast[which] through the sort.
+hipError_t learn_prev(fe_ctx* c, int which);
+// Features of the coming solve `which` into ws.feat (schedule order).
+hipError_t learn_features(fe_ctx* c, int which);
+// Predicted initial pressures of solve `which` into pressures and the p / rho^2 mirror.
+hipError_t learn_infer(fe_ctx* c, int which);
+// Append ws.per_solve samples (features, current pressures) to dataset `which`.
+hipError_t learn_collect(fe_ctx* c, int which);
+// Raw network output for standardised inputs x (n x kMlpIn, device) into y (device).
+hipError_t learn_forward_raw(fe_ctx* c, int which, const float* x, uint32_t n, float* y);
+// One training step on `batch` samples of (x, t): sample b is row b when sample_mod == 0,
+// else row hash(seed, b) % sample_mod. Sums the gradient into grad_out (if not null) and
+// the summed squared error into loss_out (if not null); applies Adam when apply != 0.
+hipError_t learn_step(fe_ctx* c, int which, const float* x, const float* t, uint32_t batch, uint32_t sample_mod,
+                      uint32_t seed, float lr, uint32_t apply, float* grad_out, float* loss_out);
 
 }  // namespace fe

@@ -14,7 +14,8 @@ from ctypes import byref, c_double, c_float, c_uint32, c_uint64, c_void_p
 import numpy as np
 
 from . import scene
-from ._ffi import (EXPORTED_SYMBOLS, FE_SOLVER_CONVERGE, FE_SOLVER_FIXED, FE_SORT_BITONIC, FE_SORT_RADIX,
+from ._ffi import (EXPORTED_SYMBOLS, FE_MLP_HIDDEN, FE_MLP_INPUTS, FE_MLP_NORM, FE_MLP_PARAMS, FE_NET_DENSITY,
+                   FE_NET_DIVERGENCE, FE_SOLVER_CONVERGE, FE_SOLVER_FIXED, FE_SORT_BITONIC, FE_SORT_RADIX,
                    FE_WARM_START_LEARNED, FE_WARM_START_REFERENCE, FIELDS, INT_FIELDS, LIB_PATH, FeOptions,
                    FeParams, FluidEngineError, check, load_library)
 
@@ -22,6 +23,7 @@ __all__ = [
     "Simulation", "FeParams", "FeOptions", "FluidEngineError", "load_library", "scene", "frame_control",
     "default_options", "FE_SORT_BITONIC", "FE_SORT_RADIX", "FE_SOLVER_FIXED", "FE_SOLVER_CONVERGE",
     "FE_WARM_START_REFERENCE", "FE_WARM_START_LEARNED", "FIELDS", "EXPORTED_SYMBOLS", "LIB_PATH",
+    "FE_NET_DENSITY", "FE_NET_DIVERGENCE", "FE_MLP_INPUTS", "FE_MLP_HIDDEN", "FE_MLP_PARAMS", "FE_MLP_NORM",
 ]
 
 
@@ -231,3 +233,63 @@ class Simulation:
                   "fe_profile_entry")
             res[buf.value.decode()] = (launches.value, total.value)
         return res
+
+    # -- learned warm start (no reference counterpart) -------------------------------------
+    def learn_collect(self, samples_per_solve: int, capacity: int = 0):
+        """Record (features, final pressure) samples after every solve; 0 stops collecting."""
+        check(self._lib.fe_learn_collect(self._ctx, samples_per_solve, capacity), "fe_learn_collect")
+
+    def learn_dataset_size(self, net: int) -> int:
+        n = c_uint32()
+        check(self._lib.fe_learn_dataset_size(self._ctx, net, byref(n)), "fe_learn_dataset_size")
+        return n.value
+
+    def learn_dataset(self, net: int, count: int | None = None):
+        count = self.learn_dataset_size(net) if count is None else count
+        x = np.empty((count, FE_MLP_INPUTS), np.float32)
+        t = np.empty(count, np.float32)
+        check(self._lib.fe_learn_dataset(self._ctx, net, x.ctypes.data, t.ctypes.data, count), "fe_learn_dataset")
+        return x, t
+
+    def learn_train(self, net: int, steps: int, batch: int = 4096, lr: float = 2e-3, seed: int = 0) -> np.ndarray:
+        """Train network `net` on its dataset; returns the per-step normalised MSE."""
+        losses = np.empty(max(steps, 1), np.float32)
+        check(self._lib.fe_learn_train(self._ctx, net, steps, batch, lr, seed, losses.ctypes.data), "fe_learn_train")
+        return losses[:steps]
+
+    def learn_weights(self, net: int):
+        p = np.empty(FE_MLP_PARAMS, np.float32)
+        nm = np.empty(FE_MLP_NORM, np.float32)
+        check(self._lib.fe_learn_get_weights(self._ctx, net, p.ctypes.data, nm.ctypes.data), "fe_learn_get_weights")
+        return p, nm
+
+    def learn_set_weights(self, net: int, params, norm):
+        p = np.ascontiguousarray(params, np.float32).reshape(FE_MLP_PARAMS)
+        nm = np.ascontiguousarray(norm, np.float32).reshape(FE_MLP_NORM)
+        check(self._lib.fe_learn_set_weights(self._ctx, net, p.ctypes.data, nm.ctypes.data), "fe_learn_set_weights")
+
+    def learn_forward(self, net: int, x) -> np.ndarray:
+        xx = np.ascontiguousarray(x, np.float32).reshape(-1, FE_MLP_INPUTS)
+        y = np.empty(xx.shape[0], np.float32)
+        check(self._lib.fe_learn_forward(self._ctx, net, xx.ctypes.data, xx.shape[0], y.ctypes.data),
+              "fe_learn_forward")
+        return y
+
+    def learn_gradient(self, net: int, x, t):
+        xx = np.ascontiguousarray(x, np.float32).reshape(-1, FE_MLP_INPUTS)
+        tt = np.ascontiguousarray(t, np.float32).reshape(xx.shape[0])
+        g = np.empty(FE_MLP_PARAMS, np.float32)
+        ss = c_float()
+        check(self._lib.fe_learn_gradient(self._ctx, net, xx.ctypes.data, tt.ctypes.data, xx.shape[0],
+                                          g.ctypes.data, byref(ss)), "fe_learn_gradient")
+        return g, ss.value
+
+    def learn_features(self, net: int):
+        """(features n x 12, previous pressures n), both in hash order."""
+        f = np.empty((self.n, FE_MLP_INPUTS), np.float32)
+        pp = np.empty(self.n, np.float32)
+        check(self._lib.fe_learn_features(self._ctx, net, f.ctypes.data, pp.ctypes.data), "fe_learn_features")
+        return f, pp
+
+    def learn_predict(self, net: int):
+        check(self._lib.fe_learn_predict(self._ctx, net), "fe_learn_predict")

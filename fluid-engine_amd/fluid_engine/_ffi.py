@@ -21,6 +21,12 @@ FE_SOLVER_FIXED = 0
 FE_SOLVER_CONVERGE = 1
 FE_WARM_START_REFERENCE = 0
 FE_WARM_START_LEARNED = 1
+FE_NET_DENSITY = 0
+FE_NET_DIVERGENCE = 1
+FE_MLP_INPUTS = 12
+FE_MLP_HIDDEN = 64
+FE_MLP_PARAMS = 9217
+FE_MLP_NORM = 26
 
 # field name -> (id, floats-or-words per particle, per-table (True) or per-particle)
 FIELDS = {
@@ -131,6 +137,16 @@ _SIGNATURES = [
     ("fe_profile_reset", [c_void_p]),
     ("fe_frame_control", [POINTER(c_uint32), c_uint32, POINTER(FeParams), POINTER(FeOptions),
                           POINTER(c_float), POINTER(c_uint32), POINTER(c_uint32)]),
+    ("fe_learn_collect", [c_void_p, c_uint32, c_uint32]),
+    ("fe_learn_dataset_size", [c_void_p, c_int, POINTER(c_uint32)]),
+    ("fe_learn_dataset", [c_void_p, c_int, c_void_p, c_void_p, c_uint32]),
+    ("fe_learn_train", [c_void_p, c_int, c_uint32, c_uint32, c_float, c_uint32, c_void_p]),
+    ("fe_learn_get_weights", [c_void_p, c_int, c_void_p, c_void_p]),
+    ("fe_learn_set_weights", [c_void_p, c_int, c_void_p, c_void_p]),
+    ("fe_learn_forward", [c_void_p, c_int, c_void_p, c_uint32, c_void_p]),
+    ("fe_learn_gradient", [c_void_p, c_int, c_void_p, c_void_p, c_uint32, c_void_p, POINTER(c_float)]),
+    ("fe_learn_features", [c_void_p, c_int, c_void_p, c_void_p]),
+    ("fe_learn_predict", [c_void_p, c_int]),
 ]
 
 EXPORTED_SYMBOLS = tuple(name for name, _ in _SIGNATURES)

@@ -75,7 +75,7 @@ typedef struct fe_params {
 /* Initial pressure of the two Jacobi solves. REFERENCE: zero before the density solve
  * (shaders/compute/density_source_term.comp:73) and the density solve's last pressure
  * before the divergence solve (pressures is never reset, src/renderer/mod.rs:323-332).
- * LEARNED: the per-particle MLP warm start (fe_warm_start_* below). */
+ * LEARNED: the per-particle MLP warm start (fe_learn_* below). */
 #define FE_WARM_START_REFERENCE 0u
 #define FE_WARM_START_LEARNED 1u
 
@@ -210,6 +210,61 @@ fe_status fe_profile_reset(fe_ctx* ctx);
 fe_status fe_frame_control(const uint32_t stats[3], uint32_t n, const fe_params* params,
                            const fe_options* options, float* dt_out,
                            uint32_t* density_iters, uint32_t* divergence_iters);
+
+/* ---- learned warm start (no reference counterpart) -----------------------------------------
+ * Two networks, FE_NET_DENSITY and FE_NET_DIVERGENCE, one per Jacobi solve. Each maps the
+ * FE_MLP_INPUTS features of a particle through three GELU layers of FE_MLP_HIDDEN units to
+ * one output, in BF16 with FP32 accumulation on the matrix cores. Features, at the start of
+ * a solve, over the particle's pairs with r^2 <= h^2 (at most 64, reference visit order):
+ *   0 (rho - rho0) / rho0          1 source_term dt / rho0        2 factor (alpha)
+ *   3 pair count                   4 previous pressure            5 sum W
+ *   6 sum W p_prev_j               7 |sum r_ij| / h               8 sum |r_ij|^2 / h^2
+ *   9 sum (v_i - v_j).r_ij / h     10 sum |v_i - v_j|^2           11 -s rho^2 / (dt alpha)
+ * with v = velocity_a and "previous pressure" the final pressure of the same particle in the
+ * previous solve of the same kind (0 on the first solve and after uploads).
+ * Parameters, FE_MLP_PARAMS floats: W1[12][64], b1[64], W2[64][64], b2[64], W3[64][64],
+ * b3[64], w4[64], b4 (W[in][out], row-major). Normalisation, FE_MLP_NORM floats: feature
+ * means[12], feature standard deviations[12], target mean, target standard deviation; the
+ * network sees (feature - mean) / std and predicts (p - target mean) / target std.
+ * With warm_start = FE_WARM_START_LEARNED each solve whose network is loaded starts from
+ * max(0, prediction); the other solve keeps the reference start. */
+#define FE_NET_DENSITY 0
+#define FE_NET_DIVERGENCE 1
+#define FE_MLP_INPUTS 12
+#define FE_MLP_HIDDEN 64
+#define FE_MLP_PARAMS 9217
+#define FE_MLP_NORM 26
+
+/* Record training samples: after every solve, samples_per_solve particles (chosen by a
+ * hash) contribute (features at the start of the solve, final pressure) to that solve's
+ * dataset, a ring buffer of `capacity` samples. samples_per_solve = 0 stops collecting but
+ * keeps the pressure history the "previous pressure" features need. */
+fe_status fe_learn_collect(fe_ctx* ctx, uint32_t samples_per_solve, uint32_t capacity);
+/* Number of samples held for network `net`. */
+fe_status fe_learn_dataset_size(fe_ctx* ctx, int net, uint32_t* size);
+/* Copy the first `count` samples of the dataset: x (count x 12, raw features), t (count). */
+fe_status fe_learn_dataset(fe_ctx* ctx, int net, float* x, float* t, uint32_t count);
+/* Train network `net` from a fresh initialisation (seeded) on its dataset: normalisation
+ * from the dataset, then `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8) on mini-batches of
+ * `batch` <= 65536 samples drawn by a hash of (seed, step, row); a batch of at least the
+ * dataset's size uses every sample once, in order. The learning rate follows a cosine from
+ * lr (first step) to lr / 20 (last step). losses (optional, `steps` floats) receives each
+ * step's mean squared error in normalised units. steps = 0 only initialises. Deterministic. */
+fe_status fe_learn_train(fe_ctx* ctx, int net, uint32_t steps, uint32_t batch, float lr, uint32_t seed,
+                         float* losses);
+fe_status fe_learn_get_weights(fe_ctx* ctx, int net, float* params, float* norm);
+fe_status fe_learn_set_weights(fe_ctx* ctx, int net, const float* params, const float* norm);
+/* Network output for n normalised inputs (n x 12 row-major) into y (n). */
+fe_status fe_learn_forward(fe_ctx* ctx, int net, const float* x, uint32_t n, float* y);
+/* Gradient of L = sum_b (y_b - t_b)^2 / n over n <= 65536 normalised samples, as the
+ * training step computes it (FE_MLP_PARAMS floats), and L * n (optional). */
+fe_status fe_learn_gradient(fe_ctx* ctx, int net, const float* x, const float* t, uint32_t n, float* grad,
+                            float* sum_sq);
+/* Features of the current state for solve `net` (n x 12, hash order) and the previous
+ * pressures (n, optional). Needs a current neighbour list (after fe_density_alpha). */
+fe_status fe_learn_features(fe_ctx* ctx, int net, float* features, float* prev_pressure);
+/* The warm start alone: features and prediction into pressures, as a learned solve begins. */
+fe_status fe_learn_predict(fe_ctx* ctx, int net);
 
 #ifdef __cplusplus
 }

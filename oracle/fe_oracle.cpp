@@ -666,6 +666,61 @@ void step(State<R>& s, float max_dt) {
 // ---------------------------------------------------------------------------------------
 // C interface.
 
+// Warm-start features (no reference counterpart; the definition the HIP path's learned warm
+// start uses): for particle i, over its accepted pairs (r^2 <= h^2) in the reference visit
+// order, at most kFeatPairs of them (the pair count itself is not capped):
+//   0 (rho - rho0) / rho0        1 s dt / rho0              2 alpha
+//   3 pair count                 4 previous pressure        5 sum W
+//   6 sum W p_prev_j             7 |sum r_ij| / h           8 sum |r_ij|^2 / h^2
+//   9 sum (v_i - v_j).r_ij / h   10 sum |v_i - v_j|^2       11 -s rho^2 / (dt alpha)
+// with v = velocity_a and p_prev the given previous pressures (hash order).
+constexpr uint32_t kFeatPairs = 64;
+
+template <class R>
+void features(const State<R>& s, const float* pprev, float* out) {
+    const R h = s.P(s.p.smoothing_radius);
+    const R hh = h * h, rho0 = s.P(s.p.target_density), dt = s.P(s.p.dt);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s.n; ii++) {
+        const uint32_t i = (uint32_t)ii;
+        const V3<R> pi = s.v3(s.pos_b, i), vi = s.v3(s.vel_a, i);
+        uint32_t count = 0;
+        R sw = R(0), swp = R(0), cx = R(0), cy = R(0), cz = R(0), sr2 = R(0), svr = R(0), sv2 = R(0);
+        for_each_candidate(s, i, [&](uint32_t j) {
+            const V3<R> rv = sub(pi, s.v3(s.pos_b, j));
+            const R r2 = dot(rv, rv);
+            if (r2 > hh) return;
+            if (++count > kFeatPairs) return;
+            const R w = kernel_w(R(std::sqrt(r2)), h);
+            const V3<R> d = sub(vi, s.v3(s.vel_a, j));
+            sw += w;
+            swp += w * R(pprev[j]);
+            cx += rv.x;
+            cy += rv.y;
+            cz += rv.z;
+            sr2 += r2;
+            svr += dot(d, rv);
+            sv2 += dot(d, d);
+        });
+        const R rho = s.dens[i], alpha = s.fac[i], si = s.src[i];
+        float* f = out + (size_t)i * 12;
+        f[0] = (float)((rho - rho0) / rho0);
+        f[1] = (float)(si * dt / rho0);
+        f[2] = (float)alpha;
+        f[3] = (float)count;
+        f[4] = pprev[i];
+        f[5] = (float)sw;
+        f[6] = (float)swp;
+        f[7] = (float)(R(std::sqrt(cx * cx + cy * cy + cz * cz)) / h);
+        f[8] = (float)(sr2 / hh);
+        f[9] = (float)(svr / h);
+        f[10] = (float)sv2;
+        f[11] = (float)((alpha > Lit<R>::eps6 && rho > Lit<R>::eps6 && dt > Lit<R>::eps6)
+                            ? -si * (rho * rho) / (dt * alpha)
+                            : R(0));
+    }
+}
+
 struct oracle_ctx {
     int fp64;
     State<float> f;
@@ -803,6 +858,9 @@ void oracle_pressure_integration(oracle_ctx* c) { DISPATCH(c, pressure_integrati
 void oracle_divergence_source(oracle_ctx* c) { DISPATCH(c, divergence_source); }
 void oracle_divergence_integration(oracle_ctx* c) { DISPATCH(c, divergence_integration); }
 void oracle_stats(oracle_ctx* c) { DISPATCH(c, stats); }
+void oracle_features(oracle_ctx* c, const float* pprev, float* out) {
+    if (c->fp64) features(c->d, pprev, out); else features(c->f, pprev, out);
+}
 void oracle_read_stats(oracle_ctx* c, uint32_t out[3]) {
     std::memcpy(out, c->fp64 ? c->d.stats : c->f.stats, 12);
 }

@@ -59,6 +59,11 @@ void oracle_divergence_integration(oracle_ctx* c);
 void oracle_stats(oracle_ctx* c);
 void oracle_read_stats(oracle_ctx* c, uint32_t out[3]);
 
+/* The 12 warm-start features of every particle (n x 12, hash order) for the current state,
+ * given the previous pressures (n, hash order). No reference counterpart: the definition is
+ * the one of include/fluid_engine.h (fe_learn_*). */
+void oracle_features(oracle_ctx* c, const float* prev_pressure, float* out);
+
 /* Jacobi solves (FIXED or CONVERGE per the options); return the sweep count. */
 uint32_t oracle_solve_density(oracle_ctx* c);
 uint32_t oracle_solve_divergence(oracle_ctx* c);

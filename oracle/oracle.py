@@ -57,6 +57,7 @@ def lib():
         "oracle_radix_sort_pairs": ([c_void_p, c_uint32], None),
         "oracle_bitonic_sort_pairs": ([c_void_p, c_uint32], None),
         "oracle_residual_q": ([c_float, c_float, c_float], c_uint64),
+        "oracle_features": ([P, c_void_p, c_void_p], None),
     }
     for stage in ("neighbor_search", "sort", "density_alpha", "viscosity", "density_source", "pressure_force",
                   "pressure_integration", "divergence_source", "divergence_integration", "stats"):
@@ -167,6 +168,13 @@ class Oracle:
         out = (c_uint32 * 3)()
         self._l.oracle_read_stats(self._c, out)
         return np.array(out[:], dtype=np.uint32)
+
+    def features(self, prev_pressure):
+        """The 12 warm-start features of every particle (n x 12, hash order)."""
+        pp = np.ascontiguousarray(prev_pressure, dtype=np.float32).reshape(self.n)
+        out = np.empty((self.n, 12), np.float32)
+        self._l.oracle_features(self._c, pp.ctypes.data, out.ctypes.data)
+        return out
 
     def last_iterations(self):
         d = c_uint32()

@@ -450,3 +450,43 @@ def test_residual_quantisation(oracle_mod):
     q = oracle_mod.residual_q(-1000.0, 1e-3, 1000.0)          # 1e-3 relative error
     assert abs(q / 2 ** 32 - 1e-3) < 1e-9
     assert oracle_mod.residual_q(-1e30, 1e-3, 1000.0) == 256 * 2 ** 32
+
+
+# ---------------------------------------------------------------------------------------
+# Warm-start features (no reference counterpart) on the lattice: closed forms for an interior
+# particle with h = 2 s (33 pairs: shells of 1, 6, 12, 8 and the 6 at exactly r = h).
+
+def test_features_lattice_closed_forms(oracle_mod):
+    w = scene.dam_break((12, 12, 12), (1.0, 1.0, 1.0), jitter_amp=0.0)
+    o = oracle_mod.Oracle(w.params, w.positions)
+    o.neighbor_search()
+    o.density_alpha()
+    v = np.zeros((w.n, 4), F)
+    v[:, :3] = [0.5, -2.0, 1.0]             # uniform velocity: no relative motion
+    o.upload("velocity_a", v)
+    prev = np.full(w.n, 2.5, F)
+    prev[0] = 7.0
+    f = o.features(prev).astype(np.float64)
+    e = o.download("grid_entries")
+    rho = o.download("densities", as_double=True)
+    m = float(F(w.params.particle_mass))
+    orig = (6 * 12 + 6) * 12 + 6            # an interior particle
+    k = int(np.nonzero(e[: w.n, 1] == orig)[0][0])
+    rho_cf, _ = lattice_closed_forms(1000.0, float(F(0.02)))
+    # the 6 neighbours at exactly r = h pass r^2 <= h^2 or not by FP32 rounding of r^2 (they
+    # contribute W = 0); each one missing removes |d|^2 / 4 = 1 from sum |r|^2 / h^2 and at
+    # most 1 from |sum r| / h
+    missing = 33 - f[k, 3]
+    assert 0 <= missing <= 6
+    assert abs(f[k, 5] * m - rho_cf) / rho_cf < 5e-6          # sum W = rho / m
+    assert abs(f[k, 6] - 2.5 * f[k, 5]) <= 1e-6 * f[k, 6]      # p_prev uniform 2.5
+    assert f[k, 4] == 2.5
+    assert abs(f[k, 7]) <= missing + 1e-5
+    assert abs(f[k, 8] - (19.5 - missing)) < 1e-4             # sum |d|^2 / 4 = 78 / 4
+    assert f[k, 9] == 0 and f[k, 10] == 0
+    assert f[k, 1] == 0 and f[k, 11] == 0                      # source terms are still 0
+    assert abs(f[k, 0] - (rho[k] - 1000.0) / 1000.0) < 1e-7
+    # the particle whose previous pressure is 7.0 has it as feature 4
+    k0 = int(np.nonzero(e[: w.n, 1] == 0)[0][0])
+    assert (f[:, 3] >= 8).all()                                # corners: 1 + 3 + 3 + 1 at least
+    assert f[k0, 4] == prev[k0]
