@@ -245,6 +245,7 @@ fe_status neighbor_search(fe_ctx* c) {
     FE_HIP(fe::launch_offsets_reorder(c), "offsets/reorder");
     c->ends_dirty = false;
     c->list_valid = false;
+    c->ws.feat_fresh = false;
     // gather schedule: particles in Morton order of their cells
     FE_HIP(fe::launch_morton_keys(c), "morton keys");
     FE_HIP(fe::radix_sort_pairs(c, c->mkeys, c->sched, c->n, c->morton_bits, "schedule"), "schedule sort");

@@ -418,6 +418,130 @@ struct DivergenceSource {
     }
 };
 
+// The warm-start features of a solve, gathered in the same pass as its source term (over
+// the same listed pairs, no extra neighbour walk): per particle, over its pairs with
+// r^2 <= h^2 in visit order (the first kMaxNbr of them; the count itself is not capped),
+//   0 (rho - rho0) / rho0        1 s dt / rho0              2 alpha
+//   3 pair count                 4 previous pressure        5 sum W
+//   6 sum W p_prev_j             7 |sum r_ij| / h           8 sum |r_ij|^2 / h^2
+//   9 sum (v_i - v_j).r_ij / h   10 sum |v_i - v_j|^2       11 -s rho^2 / (dt alpha)
+// with v the velocity the source term uses, written feature-major: feat[k n + t].
+struct FeatAcc {
+    uint32_t cnt;
+    float sw, swp, cx, cy, cz, sr2, svr, sv2;
+};
+
+__device__ __forceinline__ FeatAcc feat_zero() { return {0u, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}; }
+
+// One accepted pair (r2 <= h^2 already checked; r = sqrt(r2), dv = v_i - v_j).
+__device__ __forceinline__ void feat_pair(FeatAcc& f, const Sph& s, float rx, float ry, float rz, float r2, float r,
+                                          float dx, float dy, float dz, float pprev_j) {
+    if (++f.cnt > kMaxNbr) return;
+    const float w = spline_w(r, s);
+    f.sw += w;
+    f.swp += w * pprev_j;
+    f.cx += rx;
+    f.cy += ry;
+    f.cz += rz;
+    f.sr2 += r2;
+    f.svr += dx * rx + dy * ry + dz * rz;
+    f.sv2 += dx * dx + dy * dy + dz * dz;
+}
+
+__device__ __forceinline__ void feat_write(float* __restrict__ feat, uint32_t n, uint32_t t, const Sph& s,
+                                           const FeatAcc& f, float rho, float alpha, float src, float pprev_i,
+                                           float rho0, float dt) {
+    float v[kMlpIn];
+    v[0] = (rho - rho0) / rho0;
+    v[1] = src * dt / rho0;
+    v[2] = alpha;
+    v[3] = (float)f.cnt;
+    v[4] = pprev_i;
+    v[5] = f.sw;
+    v[6] = f.swp;
+    v[7] = sqrtf(f.cx * f.cx + f.cy * f.cy + f.cz * f.cz) / s.h;
+    v[8] = f.sr2 / s.hh;
+    v[9] = f.svr / s.h;
+    v[10] = f.sv2;
+    v[11] = (alpha > 1e-6f && rho > 1e-6f && dt > 1e-6f) ? -src * (rho * rho) / (dt * alpha) : 0.0f;
+#pragma unroll
+    for (int k = 0; k < kMlpIn; k++) feat[(size_t)k * n + t] = v[k];
+}
+
+// DensitySource plus the features of the density solve.
+struct DensitySourceFeat {
+    DensitySource base;
+    const float* fac;
+    const float* pprev_m;
+    float* feat;
+    struct Self { float4 v; float pp; };
+    struct Cand { float4 v; float pp; };
+    struct Acc { float div; FeatAcc f; };
+    __device__ Self self(uint32_t, uint32_t t) const { return {base.va_m[t], pprev_m[t]}; }
+    __device__ Cand load(uint32_t jt) const { return {base.va_m[jt], pprev_m[jt]}; }
+    __device__ Acc zero() const { return {0.f, feat_zero()}; }
+    __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 > s.hh) return;
+        const float r = sqrtf(r2);
+        const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
+        if (r > 1e-6f) {
+            const float g = exact_grad(r, s);
+            a.div += s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz));
+        }
+        feat_pair(a.f, s, rx, ry, rz, r2, r, dx, dy, dz, c.pp);
+    }
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si) const {
+        const float rho = base.dens[i];
+        float source = 0.0f;
+        if (base.dt > 1e-6f) source = ((base.rho0 - rho) / base.dt) - a.div;
+        base.src[i] = source;
+        base.pres[i] = 0.0f;
+        base.prho_m[t] = prho_of(0.0f, rho);
+        feat_write(feat, n_, t, sph_, a.f, rho, fac[i], source, si.pp, base.rho0, base.dt);
+        return 0;
+    }
+    uint32_t n_;
+    Sph sph_;
+};
+
+// DivergenceSource plus the features of the divergence solve (velocity_a read at the
+// current order's indices, as the source term reads it).
+struct DivergenceSourceFeat {
+    DivergenceSource base;
+    const float* fac;
+    const float* pprev_m;
+    float* feat;
+    float rho0, dt;
+    struct Self { float4 v; float pp; };
+    struct Cand { float4 v; float pp; };
+    struct Acc { float div; FeatAcc f; };
+    __device__ Self self(uint32_t i, uint32_t t) const { return {base.vel_a[i], pprev_m[t]}; }
+    __device__ Cand load(uint32_t jt) const { return {base.vel_a[base.sched[jt]], pprev_m[jt]}; }
+    __device__ Acc zero() const { return {0.f, feat_zero()}; }
+    __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 > s.hh) return;
+        const float r = sqrtf(r2);
+        const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
+        if (r > 1e-6f) {
+            const float g = exact_grad(r, s);
+            a.div += s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz));
+        }
+        feat_pair(a.f, s, rx, ry, rz, r2, r, dx, dy, dz, c.pp);
+    }
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si) const {
+        const float source = -a.div;
+        base.src[i] = source;
+        const float rho = base.dens[i];
+        base.prho_m[t] = prho_of(base.pres[i], rho);
+        feat_write(feat, n_, t, sph_, a.f, rho, fac[i], source, si.pp, rho0, dt);
+        return 0;
+    }
+    uint32_t n_;
+    Sph sph_;
+};
+
 // pressure_force.comp:15-78. Neighbours supply p_j / rho_j^2 (kNoRho when rho_j <= 1e-6,
 // which the reference skips).
 struct PressureForce {
@@ -667,7 +791,14 @@ hipError_t launch_density_source(fe_ctx* c) {
     hipError_t e = need(c, kMirrorPos | kMirrorVelA);
     if (e != hipSuccess) return e;
     DensitySource pol{c->va_m, c->dens, c->src, c->pres, c->prho_m, c->params.dt, c->params.target_density};
-    e = run_list(c, "density_source", pol);
+    if (learn_features_wanted(c, 0)) {
+        if ((e = learn_prev(c, 0)) != hipSuccess) return e;
+        DensitySourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n, sph_of(c)};
+        e = run_list(c, "density_source_feat", fp);
+        learn_features_done(c, 0);
+    } else {
+        e = run_list(c, "density_source", pol);
+    }
     c->mirrors |= kMirrorPrho;
     return e;
 }
@@ -676,7 +807,15 @@ hipError_t launch_divergence_source(fe_ctx* c) {
     hipError_t e = need(c, kMirrorPos);
     if (e != hipSuccess) return e;
     DivergenceSource pol{c->vel_a, c->sched[0], c->dens, c->pres, c->src, c->prho_m};
-    e = run_list(c, "divergence_source", pol);
+    if (learn_features_wanted(c, 1)) {
+        if ((e = learn_prev(c, 1)) != hipSuccess) return e;
+        DivergenceSourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->params.target_density, c->params.dt,
+                                c->n, sph_of(c)};
+        e = run_list(c, "divergence_source_feat", fp);
+        learn_features_done(c, 1);
+    } else {
+        e = run_list(c, "divergence_source", pol);
+    }
     c->mirrors |= kMirrorPrho;
     return e;
 }

@@ -48,6 +48,8 @@ constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + 2 * (kMlpHidden * 
 constexpr int kMlpNorm = 2 * kMlpIn + 2;        // feature means, feature stds, target mean, target std
 constexpr uint32_t kMlpMaxBatch = 1u << 16;     // samples per training step
 constexpr uint32_t kMlpSamplesPerBlock = 32;    // one wave of the training kernel
+constexpr int kMlpPackedOps = 20;               // 16x32 weight tiles: 4 (layer 1) + 8 + 8
+constexpr int kMlpPackedFloats = kMlpPackedOps * 64 * 4 + 288 + 2 * 2048;   // + the GELU table
 
 struct LearnState {
     bool alloc = false;
@@ -58,13 +60,20 @@ struct LearnState {
     float* norm[2] = {nullptr, nullptr};
     bool ready[2] = {false, false};
     uint32_t adam_t[2] = {0, 0};
-    // per particle: features of the coming solve (schedule order), the previous pressure of
-    // each particle in the current hash order, and the last solve's final pressures (hash
-    // order of that solve)
+    // inference copy of each network: BF16 weight operands in the matrix-core lane layout,
+    // then biases, the output layer and the normalisation (kMlpPackedFloats floats)
+    float* packed[2] = {nullptr, nullptr};
+    // per particle: features of the coming solve (feature-major, feat[k n + t] for schedule
+    // position t), the previous pressure of each particle by schedule position, and the last
+    // solve's final pressures (hash order of that solve)
     float* feat = nullptr;
     float* pprev = nullptr;
     float* plast[2] = {nullptr, nullptr};
     bool plast_valid[2] = {false, false};
+    // the source-term stage of solve feat_kind gathered features that are still current
+    bool feat_fresh = false;
+    int feat_kind = -1;
+    bool force_features = false;
     // dataset (ring buffer per network) of (features, final pressure)
     float* data_x[2] = {nullptr, nullptr};
     float* data_t[2] = {nullptr, nullptr};
@@ -229,16 +238,21 @@ hipError_t launch_pack_start(fe_ctx* c, uint32_t* out);
 hipError_t launch_unpack_start(fe_ctx* c, const uint32_t* in);
 
 // ---- launchers (fe_learn.hip) ----
-// Previous pressure of every particle (ws.pprev) from ws.plast[which] through the sort.
+// Previous pressure of every particle by schedule position (ws.pprev) from ws.plast[which]
+// through the sort.
 hipError_t learn_prev(fe_ctx* c, int which);
-// Features of the coming solve `which` into ws.feat (schedule order).
-hipError_t learn_features(fe_ctx* c, int which);
+// Inference copy of network `which` (ws.packed) from its parameters and normalisation.
+hipError_t learn_pack(fe_ctx* c, int which);
 // Predicted initial pressures of solve `which` into pressures and the p / rho^2 mirror.
 hipError_t learn_infer(fe_ctx* c, int which);
 // Append ws.per_solve samples (features, current pressures) to dataset `which`.
 hipError_t learn_collect(fe_ctx* c, int which);
-// Raw network output for standardised inputs x (n x kMlpIn, device) into y (device).
+// Raw network output for standardised inputs x (feature-major, x[k n + b], device) into y.
 hipError_t learn_forward_raw(fe_ctx* c, int which, const float* x, uint32_t n, float* y);
+// (fe_learn_api.cpp) Whether the source-term stage of solve `kind` also gathers the
+// features, and the record that it did.
+bool learn_features_wanted(const fe_ctx* c, int kind);
+void learn_features_done(fe_ctx* c, int kind);
 // One training step on `batch` samples of (x, t): sample b is row b when sample_mod == 0,
 // else row hash(seed, b) % sample_mod. Sums the gradient into grad_out (if not null) and
 // the summed squared error into loss_out (if not null); applies Adam when apply != 0.

@@ -40,9 +40,30 @@ __device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU(z) = z Phi(z), Phi(z) = (1 + erf(z / sqrt 2)) / 2, with erf from Abramowitz & Stegun
+// 7.1.26 (Hastings' rational approximation, |error| <= 1.5e-7) on the hardware exponential
+// and reciprocal: one exp2 and one rcp per activation. e = exp(-z^2 / 2) is returned for the
+// derivative GELU'(z) = Phi(z) + z e / sqrt(2 pi).
+__device__ __forceinline__ float phi_of(float z, float& e) {
+    const float x = fabsf(z) * 0.70710678118654752f;
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, x, 1.0f));
+    float p = fmaf(1.061405429f, t, -1.453152027f);
+    p = fmaf(p, t, 1.421413741f);
+    p = fmaf(p, t, -0.284496736f);
+    p = fmaf(p, t, 0.254829592f);
+    p *= t;
+    e = __builtin_amdgcn_exp2f(z * z * -0.72134752044448170f);
+    const float half_erfc = 0.5f * p * e;   // (1 - erf(|z| / sqrt 2)) / 2
+    return z >= 0.0f ? 1.0f - half_erfc : half_erfc;
+}
+__device__ __forceinline__ float gelu(float z) {
+    float e;
+    return z * phi_of(z, e);
+}
 __device__ __forceinline__ float gelu_grad(float z) {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+    float e;
+    const float ph = phi_of(z, e);
+    return fmaf(z * 0.39894228040143268f, e, ph);
 }
 __device__ __forceinline__ __bf16 tobf(float x) { return (__bf16)x; }
 
@@ -143,86 +164,212 @@ __device__ __forceinline__ void forward16(const FwdWeights& w, const __bf16* x, 
     }
 }
 
-// Output layer of a 16-row tile whose last activations are in h (BF16 [16][H]): row
-// lane % 16, 16 inputs per lane, summed over the four lanes of the row. Every lane returns
-// its row's value.
-__device__ __forceinline__ float out_layer16(const FwdWeights& w, const __bf16* h) {
-    const int l = threadIdx.x & 63;
-    const int r = l & 15, q = l >> 4;
-    float y = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 16; k++) y += (float)h[r * H + q * 16 + k] * w.w4[q * 16 + k];
-    y += __shfl_xor(y, 16, 64);
-    y += __shfl_xor(y, 32, 64);
-    return y + w.b4;
-}
-
 // ---- inference -------------------------------------------------------------------------
+//
+// Transposed products, Z^T = W^T H^T: the weights are the A operand (16 output neurons x
+// 32 inputs per tile) and stay in registers for the whole launch; the activations of 16
+// samples are the B operand (32 inputs x 16 samples). The accumulator then gives lane
+// c + 16 q the neurons 4q .. 4q + 3 of sample c, which are contiguous in a [sample][neuron]
+// activation row (one 8-byte LDS store per tile), and the layer-1 operand (features 8q ..
+// 8q + 7 of sample c) is built in registers from the feature-major input.
 
-constexpr int kInferWaves = 4;
+// Packed inference copy, in floats: kMlpPackedOps x 64 lanes x 8 BF16 weight operands, then
+// b1, b2, b3, the BF16-rounded w4, b4, the feature means and reciprocal standard deviations,
+// the target mean and standard deviation.
+constexpr int PK_B1 = kMlpPackedOps * 64 * 4, PK_B2 = PK_B1 + H, PK_B3 = PK_B2 + H, PK_W4 = PK_B3 + H,
+              PK_B4 = PK_W4 + H, PK_MEAN = PK_B4 + 1, PK_ISTD = PK_MEAN + kMlpIn, PK_TMEAN = PK_ISTD + kMlpIn,
+              PK_TSTD = PK_TMEAN + 1, PK_END = PK_TSTD + 1;
+static_assert(PK_END <= kMlpPackedFloats, "packed layout");
 
-struct InferLds {
-    FwdWeights w;
-    alignas(16) __bf16 x[kInferWaves][16][KIN];
-    alignas(16) __bf16 h[kInferWaves][2][16][H];
-};
+// GELU of the inference kernel: piecewise linear through GELU at z = -8 + k / 128,
+// k = 0 .. 2048 (entry k holds the value at node k and the rise to node k + 1), with the end
+// segments extended linearly (GELU(z) is 0 below -8 and z above 8 to within 1e-14).
+// Interpolation error <= (1/128)^2 / 8 * max|GELU''| < 7e-6, far below a BF16 step of the
+// activations it feeds; the table is built from gelu() above.
+constexpr int kGeluNodes = 2048;
+constexpr float kGeluLo = -8.0f, kGeluScale = 128.0f;
 
-// p = max(0, y * target_std + target_mean) with y = MLP((features - mean) / std); written
-// to pressures (hash order) and to the p / rho^2 mirror (schedule order).
-__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ params, const float* __restrict__ norm,
-                                                   const float* __restrict__ feat, const uint32_t* __restrict__ sched,
-                                                   const float* __restrict__ dens, uint32_t n, float* __restrict__ pres,
-                                                   float* __restrict__ prho_m) {
-    __shared__ InferLds s;
-    load_fwd(params, s.w);
-    __syncthreads();
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t tiles = (n + 15) / 16;
-    for (uint32_t tile = blockIdx.x * kInferWaves + w; tile < tiles; tile += gridDim.x * kInferWaves) {
-        const uint32_t r0 = tile * 16;
-        for (int e = l; e < 16 * KIN; e += 64) {
-            const int r = e / KIN, k = e % KIN;
-            const uint32_t t = r0 + r;
-            float v = 0.0f;
-            if (k < kMlpIn && t < n) v = (feat[(size_t)t * kMlpIn + k] - norm[k]) / norm[kMlpIn + k];
-            s.x[w][r][k] = tobf(v);
-        }
-        wave_lds_fence();
-        forward16<false>(s.w, &s.x[w][0][0], &s.h[w][0][0][0], &s.h[w][1][0][0], &s.h[w][0][0][0], nullptr);
-        const float y = out_layer16(s.w, &s.h[w][0][0][0]);
-        const uint32_t t = r0 + (l & 15);
-        if (l < 16 && t < n) {
-            const uint32_t i = sched[t];
-            float p = y * norm[2 * kMlpIn + 1] + norm[2 * kMlpIn];
-            p = p > 0.0f ? p : 0.0f;
-            pres[i] = p;
-            prho_m[t] = prho_of(p, dens[i]);
-        }
-        wave_lds_fence();
+__global__ __launch_bounds__(256) void k_gelu_table(float2* __restrict__ table) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < kGeluNodes; k += gridDim.x * blockDim.x) {
+        const double z0 = (double)kGeluLo + (double)k / kGeluScale, z1 = z0 + 1.0 / kGeluScale;
+        const float g0 = gelu((float)z0), g1 = gelu((float)z1);
+        table[k] = make_float2(g0, g1 - g0);
     }
 }
 
-// Raw network output for already standardised inputs (no clamp, no de-standardisation).
-__global__ __launch_bounds__(256) void k_mlp_forward_raw(const float* __restrict__ params,
-                                                         const float* __restrict__ x, uint32_t n,
-                                                         float* __restrict__ y) {
-    __shared__ InferLds s;
-    load_fwd(params, s.w);
-    __syncthreads();
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t tiles = (n + 15) / 16;
-    for (uint32_t tile = blockIdx.x * kInferWaves + w; tile < tiles; tile += gridDim.x * kInferWaves) {
-        const uint32_t r0 = tile * 16;
-        for (int e = l; e < 16 * KIN; e += 64) {
-            const int r = e / KIN, k = e % KIN;
-            const uint32_t t = r0 + r;
-            s.x[w][r][k] = tobf(k < kMlpIn && t < n ? x[(size_t)t * kMlpIn + k] : 0.0f);
+static_assert(PK_END + 2 * kGeluNodes <= kMlpPackedFloats, "packed layout with the GELU table");
+
+__device__ __forceinline__ float gelu_lut(const float2* __restrict__ lut, float z) {
+    const float u = fmaf(z, kGeluScale, -kGeluLo * kGeluScale);
+    int k = (int)floorf(u);   // v_cvt_i32_f32: NaN -> 0, saturating; clamped below in any case
+    k = k < 0 ? 0 : (k > kGeluNodes - 1 ? kGeluNodes - 1 : k);
+    const float2 e = lut[k];
+    return fmaf(e.y, u - (float)k, e.x);
+}
+
+// Operand g: layer 1 tiles g = 0..3 (column tile ct = g), layer 2 g = 4 + 2 ct + kc, layer 3
+// g = 12 + 2 ct + kc. Lane l holds W[kc 32 + 8 (l / 16) + e][ct 16 + l % 16], e = 0..7.
+__global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ params, const float* __restrict__ norm,
+                                                  float* __restrict__ packed) {
+    __bf16* ops = reinterpret_cast<__bf16*>(packed);
+    for (int e = threadIdx.x; e < kMlpPackedOps * 64 * 8; e += blockDim.x) {
+        const int g = e / 512, l = (e / 8) % 64, el = e % 8;
+        int layer, ct, kc;
+        if (g < 4) {
+            layer = 0; ct = g; kc = 0;
+        } else {
+            layer = g < 12 ? 1 : 2;
+            ct = ((g - 4) % 8) / 2;
+            kc = (g - 4) % 2;
         }
+        const int in = kc * 32 + 8 * (l >> 4) + el, out = ct * 16 + (l & 15);
+        float v = 0.0f;
+        if (layer == 0) v = in < kMlpIn ? params[Off::W1 + in * H + out] : 0.0f;
+        else v = params[(layer == 1 ? Off::W2 : Off::W3) + in * H + out];
+        ops[e] = tobf(v);
+    }
+    for (int e = threadIdx.x; e < H; e += blockDim.x) {
+        packed[PK_B1 + e] = params[Off::B1 + e];
+        packed[PK_B2 + e] = params[Off::B2 + e];
+        packed[PK_B3 + e] = params[Off::B3 + e];
+        packed[PK_W4 + e] = (float)tobf(params[Off::W4 + e]);
+    }
+    for (int e = threadIdx.x; e < kMlpIn; e += blockDim.x) {
+        packed[PK_MEAN + e] = norm[e];
+        packed[PK_ISTD + e] = 1.0f / norm[kMlpIn + e];
+    }
+    if (threadIdx.x == 0) {
+        packed[PK_B4] = params[Off::B4];
+        packed[PK_TMEAN] = norm[2 * kMlpIn];
+        packed[PK_TSTD] = norm[2 * kMlpIn + 1];
+    }
+}
+
+constexpr int kInferWaves = 4;
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 u) { return __builtin_bit_cast(bf16x8, u); }
+
+// The 8 inputs lane (c, q) feeds for sample s: features 8q .. 8q + 7 (q < 2), zero otherwise.
+__device__ __forceinline__ void load_inputs(const float* __restrict__ feat, uint32_t n, uint32_t s, int q,
+                                            float (&x)[8]) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int k = 8 * q + e;
+        x[e] = (q < 2 && k < kMlpIn && s < n) ? feat[(size_t)k * n + s] : 0.0f;
+    }
+}
+
+// One layer's bias + GELU on the accumulators of the four column tiles, written as BF16
+// activation rows act[sample][neuron].
+__device__ __forceinline__ void store_act(const float2* lut, const float* sb, const f32x4 (&acc)[4], __bf16* act,
+                                          int c, int q) {
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) {
+        const float4 b = *reinterpret_cast<const float4*>(sb + ct * 16 + 4 * q);
+        const __bf16 h0 = tobf(gelu_lut(lut, acc[ct][0] + b.x)), h1 = tobf(gelu_lut(lut, acc[ct][1] + b.y));
+        const __bf16 h2 = tobf(gelu_lut(lut, acc[ct][2] + b.z)), h3 = tobf(gelu_lut(lut, acc[ct][3] + b.w));
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        const bf16x4 v = {h0, h1, h2, h3};
+        *reinterpret_cast<bf16x4*>(act + c * H + ct * 16 + 4 * q) = v;
+    }
+}
+
+// Per sample s < n (s = schedule position): y = MLP(standardised features). With raw != 0,
+// y goes to yraw[s] and the inputs are used as given; otherwise p = max(0, y target_std +
+// target_mean) goes to pressures[sched[s]] and p / rho^2 to the mirror prho_m[s].
+__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ packed,
+                                                   const float2* __restrict__ gelu_table,
+                                                   const float* __restrict__ feat, uint32_t n, uint32_t raw,
+                                                   const uint32_t* __restrict__ sched, const float* __restrict__ dens,
+                                                   float* __restrict__ pres, float* __restrict__ prho_m,
+                                                   float* __restrict__ yraw) {
+    __shared__ alignas(16) float sf[PK_END - PK_B1];
+    __shared__ alignas(16) __bf16 act[kInferWaves][16 * H];
+    __shared__ float2 lut[kGeluNodes];
+    for (int e = threadIdx.x; e < PK_END - PK_B1; e += blockDim.x) sf[e] = packed[PK_B1 + e];
+    for (int e = threadIdx.x; e < kGeluNodes; e += blockDim.x) lut[e] = gelu_table[e];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = l & 15, q = l >> 4;
+    const uint4* ops = reinterpret_cast<const uint4*>(packed);
+    bf16x8 A1[4], A2[8], A3[8];
+#pragma unroll
+    for (int g = 0; g < 4; g++) A1[g] = as_bf16x8(ops[g * 64 + l]);
+#pragma unroll
+    for (int g = 0; g < 8; g++) {
+        A2[g] = as_bf16x8(ops[(4 + g) * 64 + l]);
+        A3[g] = as_bf16x8(ops[(12 + g) * 64 + l]);
+    }
+    __syncthreads();
+    float mu[8], is[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int k = 8 * q + e;
+        const bool ok = q < 2 && k < kMlpIn;
+        mu[e] = ok && !raw ? sf[PK_MEAN - PK_B1 + k] : 0.0f;
+        is[e] = ok ? (raw ? 1.0f : sf[PK_ISTD - PK_B1 + k]) : 0.0f;
+    }
+    const float b4 = sf[PK_B4 - PK_B1], tmean = sf[PK_TMEAN - PK_B1], tstd = sf[PK_TSTD - PK_B1];
+    __bf16* a = act[w];
+    const uint32_t tiles = (n + 15) / 16, stride = gridDim.x * kInferWaves;
+    uint32_t tile = blockIdx.x * kInferWaves + w;
+    float nx[8];
+    load_inputs(feat, n, tile * 16 + c, q, nx);
+    for (; tile < tiles; tile += stride) {
+        float x[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) x[e] = nx[e];
+        if (tile + stride < tiles) load_inputs(feat, n, (tile + stride) * 16 + c, q, nx);   // next tile in flight
+        bf16x8 B;
+#pragma unroll
+        for (int e = 0; e < 8; e++) B[e] = tobf((x[e] - mu[e]) * is[e]);
+        f32x4 acc[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) acc[ct] = mfma(A1[ct], B, f32x4{0.f, 0.f, 0.f, 0.f});
+        store_act(lut, sf + (PK_B1 - PK_B1), acc, a, c, q);
         wave_lds_fence();
-        forward16<false>(s.w, &s.x[w][0][0], &s.h[w][0][0][0], &s.h[w][1][0][0], &s.h[w][0][0][0], nullptr);
-        const float v = out_layer16(s.w, &s.h[w][0][0][0]);
-        if (l < 16 && r0 + l < n) y[r0 + l] = v;
-        wave_lds_fence();
+#pragma unroll
+        for (int layer = 0; layer < 2; layer++) {
+            const bf16x8 B0 = *reinterpret_cast<const bf16x8*>(a + c * H + 8 * q);
+            const bf16x8 B1 = *reinterpret_cast<const bf16x8*>(a + c * H + 32 + 8 * q);
+            wave_lds_fence();   // both operands read before the row is overwritten
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++) {
+                const bf16x8& a0 = layer == 0 ? A2[2 * ct] : A3[2 * ct];
+                const bf16x8& a1 = layer == 0 ? A2[2 * ct + 1] : A3[2 * ct + 1];
+                acc[ct] = mfma(a1, B1, mfma(a0, B0, f32x4{0.f, 0.f, 0.f, 0.f}));
+            }
+            if (layer == 0) {
+                store_act(lut, sf + (PK_B2 - PK_B1), acc, a, c, q);
+                wave_lds_fence();
+            }
+        }
+        // output layer from the layer-3 accumulators: this lane's 16 neurons, then the four
+        // lanes of the sample
+        float y = 0.0f;
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) {
+            const float4 b = *reinterpret_cast<const float4*>(sf + (PK_B3 - PK_B1) + ct * 16 + 4 * q);
+            const float4 w4 = *reinterpret_cast<const float4*>(sf + (PK_W4 - PK_B1) + ct * 16 + 4 * q);
+            y += (float)tobf(gelu_lut(lut, acc[ct][0] + b.x)) * w4.x;
+            y += (float)tobf(gelu_lut(lut, acc[ct][1] + b.y)) * w4.y;
+            y += (float)tobf(gelu_lut(lut, acc[ct][2] + b.z)) * w4.z;
+            y += (float)tobf(gelu_lut(lut, acc[ct][3] + b.w)) * w4.w;
+        }
+        y += __shfl_xor(y, 16, 64);
+        y += __shfl_xor(y, 32, 64);
+        y += b4;
+        const uint32_t s = tile * 16 + c;
+        if (q == 0 && s < n) {
+            if (raw) {
+                yraw[s] = y;
+            } else {
+                const uint32_t i = sched[s];
+                float p = y * tstd + tmean;
+                p = p > 0.0f ? p : 0.0f;
+                pres[i] = p;
+                prho_m[s] = prho_of(p, dens[i]);
+            }
+        }
     }
 }
 
@@ -411,77 +558,22 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ params, fl
     }
 }
 
-// ---- features and samples ------------------------------------------------------------------
+// ---- previous pressures and samples ---------------------------------------------------------
 
-// pprev[i] = plast[vals[i]]: the last solve's pressure of the particle now at hash index i
-// (vals maps the current sort order to the order position_a had, which is the order of the
-// previous solve of the same kind).
-__global__ __launch_bounds__(kBlock) void k_prev(const uint32_t* __restrict__ vals, const float* __restrict__ plast,
-                                                 uint32_t valid, uint32_t n, float* __restrict__ pprev) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t v = vals[i];
-    pprev[i] = (valid && v < n) ? plast[v] : 0.0f;   // v >= n: no search has run yet
-}
-
-// The 12 features of the particle at schedule position t, over its listed pairs
-// (r^2 <= h^2 in the reference's visit order; the first kMaxNbr of them):
-//   0 (rho - rho0) / rho0        1 s dt / rho0              2 alpha
-//   3 pair count                 4 previous pressure        5 sum W
-//   6 sum W p_prev_j             7 |sum r_ij| / h           8 sum |r_ij|^2 / h^2
-//   9 sum (v_i - v_j).r_ij / h   10 sum |v_i - v_j|^2       11 -s rho^2 / (dt alpha)
-// with v = velocity_a (hash order) and W the reference kernel.
-__global__ __launch_bounds__(kBlock) void k_features(Sph s, const uint32_t* __restrict__ sched,
-                                                     const float4* __restrict__ pos_m, const uint32_t* __restrict__ nbr,
-                                                     const uint32_t* __restrict__ cnt, const float4* __restrict__ vel_a,
-                                                     const float* __restrict__ dens, const float* __restrict__ fac,
-                                                     const float* __restrict__ src, const float* __restrict__ pprev,
-                                                     float rho0, float dt, float* __restrict__ feat) {
+// pprev[t] = plast[vals[sched[t]]]: the last solve's pressure of the particle at schedule
+// position t (vals maps the current sort order to the order position_a had, which is the
+// order of the previous solve of the same kind).
+__global__ __launch_bounds__(kBlock) void k_prev(const uint32_t* __restrict__ sched, const uint32_t* __restrict__ vals,
+                                                 const float* __restrict__ plast, uint32_t valid, uint32_t n,
+                                                 float* __restrict__ pprev) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= s.n) return;
-    const uint32_t i = sched[t];
-    const float4 pi = pos_m[t];
-    const float4 vi = vel_a[i];
-    const uint32_t count = cnt[t];
-    const uint32_t m = count < kMaxNbr ? count : kMaxNbr;
-    float sw = 0.f, swp = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, sr2 = 0.f, svr = 0.f, sv2 = 0.f;
-    const uint4* tile = list_tile(nbr, t);
-    for (uint32_t k = 0; k < m; k++) {
-        const uint32_t jt = list_pick(tile[(k >> 2) * 16], k);
-        const float4 pj = pos_m[jt];
-        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const float w = exact_w(sqrtf(r2), s);
-        const uint32_t j = sched[jt];
-        const float4 vj = vel_a[j];
-        const float dx = vi.x - vj.x, dy = vi.y - vj.y, dz = vi.z - vj.z;
-        sw += w;
-        swp += w * pprev[j];
-        cx += rx;
-        cy += ry;
-        cz += rz;
-        sr2 += r2;
-        svr += dx * rx + dy * ry + dz * rz;
-        sv2 += dx * dx + dy * dy + dz * dz;
-    }
-    const float rho = dens[i], alpha = fac[i], si = src[i];
-    float* f = feat + (size_t)t * kMlpIn;
-    f[0] = (rho - rho0) / rho0;
-    f[1] = si * dt / rho0;
-    f[2] = alpha;
-    f[3] = (float)count;
-    f[4] = pprev[i];
-    f[5] = sw;
-    f[6] = swp;
-    f[7] = sqrtf(cx * cx + cy * cy + cz * cz) / s.h;
-    f[8] = sr2 / s.hh;
-    f[9] = svr / s.h;
-    f[10] = sv2;
-    f[11] = (alpha > 1e-6f && rho > 1e-6f && dt > 1e-6f) ? -si * (rho * rho) / (dt * alpha) : 0.0f;
+    if (t >= n) return;
+    const uint32_t v = vals[sched[t]];
+    pprev[t] = (valid && v < n) ? plast[v] : 0.0f;   // v >= n: no search has run yet
 }
 
 // Append `count` samples (features, pressure) of particles picked by a hash of (salt, k) to
-// a ring buffer of `cap` samples starting at `head`.
+// a ring buffer of `cap` samples starting at `head` (rows of kMlpIn features).
 __global__ __launch_bounds__(kBlock) void k_collect(const float* __restrict__ feat, const uint32_t* __restrict__ sched,
                                                     const float* __restrict__ pres, uint32_t n, uint32_t count,
                                                     uint32_t salt, uint32_t head, uint32_t cap, float* __restrict__ dx,
@@ -490,42 +582,47 @@ __global__ __launch_bounds__(kBlock) void k_collect(const float* __restrict__ fe
     if (k >= count) return;
     const uint32_t t = sample_row(k, n, salt);
     const uint32_t slot = (head + k) % cap;
-    for (int e = 0; e < kMlpIn; e++) dx[(size_t)slot * kMlpIn + e] = feat[(size_t)t * kMlpIn + e];
+    for (int e = 0; e < kMlpIn; e++) dx[(size_t)slot * kMlpIn + e] = feat[(size_t)e * n + t];
     dt[slot] = pres[sched[t]];
 }
 
 inline dim3 grid_for(uint32_t count) { return dim3((count + kBlock - 1) / kBlock); }
 
+// Persistent grid of the inference kernel: 4 waves per block, about 8 waves per CU.
 inline uint32_t infer_blocks(uint32_t n) {
     const uint32_t tiles = (n + 15) / 16;
     const uint32_t blocks = (tiles + kInferWaves - 1) / kInferWaves;
-    return blocks > 1024 ? 1024 : (blocks ? blocks : 1);
+    return blocks > 512 ? 512 : (blocks ? blocks : 1);
 }
 
 }  // namespace
 
 hipError_t learn_prev(fe_ctx* c, int which) {
     Launch L(c, "ws_prev");
-    hipLaunchKernelGGL(k_prev, grid_for(c->n), dim3(kBlock), 0, c->stream, c->vals[0], c->ws.plast[which],
-                       c->ws.plast_valid[which] ? 1u : 0u, c->n, c->ws.pprev);
+    hipLaunchKernelGGL(k_prev, grid_for(c->n), dim3(kBlock), 0, c->stream, c->sched[0], c->vals[0],
+                       c->ws.plast[which], c->ws.plast_valid[which] ? 1u : 0u, c->n, c->ws.pprev);
     return hipGetLastError();
 }
 
-hipError_t learn_features(fe_ctx* c, int which) {
-    hipError_t e = learn_prev(c, which);
+hipError_t learn_pack(fe_ctx* c, int which) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(256), 0, c->stream, c->ws.params[which], c->ws.norm[which],
+                       c->ws.packed[which]);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    Launch L(c, "ws_features");
-    hipLaunchKernelGGL(k_features, grid_for(c->n), dim3(kBlock), 0, c->stream, sph_of(c), c->sched[0], c->pos_m,
-                       c->nbr, c->nbr_count, c->vel_a, c->dens, c->fac, c->src, c->ws.pprev,
-                       c->params.target_density, c->params.dt, c->ws.feat);
+    hipLaunchKernelGGL(k_gelu_table, dim3(kGeluNodes / 256), dim3(256), 0, c->stream,
+                       reinterpret_cast<float2*>(c->ws.packed[which] + kMlpPackedFloats - 2 * kGeluNodes));
     return hipGetLastError();
+}
+
+inline const float2* gelu_table_of(const fe_ctx* c, int which) {
+    return reinterpret_cast<const float2*>(c->ws.packed[which] + kMlpPackedFloats - 2 * kGeluNodes);
 }
 
 hipError_t learn_infer(fe_ctx* c, int which) {
     Launch L(c, "ws_infer");
     hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(c->n)), dim3(64 * kInferWaves), 0, c->stream,
-                       c->ws.params[which], c->ws.norm[which], c->ws.feat, c->sched[0], c->dens, c->n, c->pres,
-                       c->prho_m);
+                       c->ws.packed[which], gelu_table_of(c, which), c->ws.feat, c->n, 0u, c->sched[0], c->dens,
+                       c->pres, c->prho_m, (float*)nullptr);
     return hipGetLastError();
 }
 
@@ -544,8 +641,9 @@ hipError_t learn_collect(fe_ctx* c, int which) {
 
 hipError_t learn_forward_raw(fe_ctx* c, int which, const float* x, uint32_t n, float* y) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mlp_forward_raw, dim3(infer_blocks(n)), dim3(64 * kInferWaves), 0, c->stream,
-                       c->ws.params[which], x, n, y);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(n)), dim3(64 * kInferWaves), 0, c->stream, c->ws.packed[which],
+                       gelu_table_of(c, which), x, n, 1u, (const uint32_t*)nullptr, (const float*)nullptr,
+                       (float*)nullptr, (float*)nullptr, y);
     return hipGetLastError();
 }
 

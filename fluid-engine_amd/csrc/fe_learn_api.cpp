@@ -41,6 +41,7 @@ fe_status ensure_alloc(fe_ctx* c) {
         FE_HIP(hipMalloc(&w.m1[k], sizeof(float) * kMlpParams), "hipMalloc adam");
         FE_HIP(hipMalloc(&w.m2[k], sizeof(float) * kMlpParams), "hipMalloc adam");
         FE_HIP(hipMalloc(&w.norm[k], sizeof(float) * kMlpNorm), "hipMalloc norm");
+        FE_HIP(hipMalloc(&w.packed[k], sizeof(float) * fe::kMlpPackedFloats), "hipMalloc packed weights");
     }
     const uint32_t parts = fe::kMlpMaxBatch / fe::kMlpSamplesPerBlock;
     FE_HIP(hipMalloc(&w.grads, sizeof(float) * kMlpParams * (size_t)parts), "hipMalloc grads");
@@ -113,7 +114,22 @@ namespace fe {
 
 bool learn_ready(const fe_ctx* c, int kind) { return c->ws.alloc && c->ws.ready[kind]; }
 
-void learn_invalidate(fe_ctx* c) { c->ws.plast_valid[0] = c->ws.plast_valid[1] = false; }
+void learn_invalidate(fe_ctx* c) {
+    c->ws.plast_valid[0] = c->ws.plast_valid[1] = false;
+    c->ws.feat_fresh = false;
+}
+
+bool learn_features_wanted(const fe_ctx* c, int kind) {
+    const LearnState& w = c->ws;
+    if (!w.alloc) return false;
+    return w.force_features || (c->options.warm_start == FE_WARM_START_LEARNED && w.ready[kind]) ||
+           (w.per_solve > 0 && w.capacity > 0);
+}
+
+void learn_features_done(fe_ctx* c, int kind) {
+    c->ws.feat_fresh = true;
+    c->ws.feat_kind = kind;
+}
 
 void learn_free(fe_ctx* c) {
     LearnState& w = c->ws;
@@ -121,32 +137,34 @@ void learn_free(fe_ctx* c) {
                     (void*)w.params[1], (void*)w.m1[0], (void*)w.m1[1], (void*)w.m2[0], (void*)w.m2[1],
                     (void*)w.norm[0], (void*)w.norm[1], (void*)w.data_x[0], (void*)w.data_x[1], (void*)w.data_t[0],
                     (void*)w.data_t[1], (void*)w.train_x, (void*)w.train_t, (void*)w.grads, (void*)w.loss,
-                    (void*)w.io})
+                    (void*)w.io, (void*)w.packed[0], (void*)w.packed[1]})
         if (p) (void)hipFree(p);
     w = LearnState{};
 }
 
+// The features come from the solve's source-term stage (fe_density_source /
+// fe_divergence_source), which gathers them in the same pass whenever they are wanted.
 fe_status learn_before_solve(fe_ctx* c, int kind) {
     LearnState& w = c->ws;
     if (!w.alloc) return FE_OK;
-    const bool use_net = c->options.warm_start == FE_WARM_START_LEARNED && w.ready[kind];
-    const bool collect = w.per_solve > 0 && w.capacity > 0;
-    if (!use_net && !collect) return FE_OK;
-    if (!c->list_valid)
+    if (c->options.warm_start != FE_WARM_START_LEARNED || !w.ready[kind]) return FE_OK;
+    if (!w.feat_fresh || w.feat_kind != kind)
         return fail(FE_ERR_INVALID_ARGUMENT,
-                    "the learned warm start needs the neighbour list of the current positions (fe_density_alpha)");
-    FE_HIP(learn_features(c, kind), "warm-start features");
-    if (use_net) FE_HIP(learn_infer(c, kind), "warm-start inference");
+                    "the learned warm start needs the features of this solve's source-term stage: run "
+                    "fe_density_source / fe_divergence_source (with warm_start LEARNED set) right before the solve");
+    FE_HIP(learn_infer(c, kind), "warm-start inference");
     return FE_OK;
 }
 
 fe_status learn_after_solve(fe_ctx* c, int kind) {
     LearnState& w = c->ws;
     if (!w.alloc) return FE_OK;
-    if (w.per_solve > 0 && w.capacity > 0) FE_HIP(learn_collect(c, kind), "warm-start samples");
+    if (w.per_solve > 0 && w.capacity > 0 && w.feat_fresh && w.feat_kind == kind)
+        FE_HIP(learn_collect(c, kind), "warm-start samples");
     FE_HIP(hipMemcpyAsync(w.plast[kind], c->pres, sizeof(float) * c->n, hipMemcpyDeviceToDevice, c->stream),
            "pressure history");
     w.plast_valid[kind] = true;
+    w.feat_fresh = false;
     return FE_OK;
 }
 
@@ -274,6 +292,11 @@ fe_status fe_learn_train(fe_ctx* c, int net, uint32_t steps, uint32_t batch, flo
         if (e != hipSuccess) st = hip_fail(e, "training wait");
     }
     if (dloss) (void)hipFree(dloss);
+    if (st == FE_OK) {
+        hipError_t e = fe::learn_pack(c, net);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) st = hip_fail(e, "weight packing");
+    }
     if (st == FE_OK) w.ready[net] = true;   // steps = 0 leaves the initialisation
     return st;
 }
@@ -309,6 +332,7 @@ fe_status fe_learn_set_weights(fe_ctx* c, int net, const float* params, const fl
            "norm upload");
     FE_HIP(hipMemsetAsync(c->ws.m1[net], 0, sizeof(float) * kMlpParams, c->stream), "adam reset");
     FE_HIP(hipMemsetAsync(c->ws.m2[net], 0, sizeof(float) * kMlpParams, c->stream), "adam reset");
+    FE_HIP(fe::learn_pack(c, net), "weight packing");
     FE_HIP(hipStreamSynchronize(c->stream), "weights wait");
     c->ws.adam_t[net] = 0;
     c->ws.ready[net] = true;
@@ -326,7 +350,11 @@ fe_status fe_learn_forward(fe_ctx* c, int net, const float* x, uint32_t n, float
     if (s != FE_OK) return s;
     float* dx = c->ws.io;
     float* dy = dx + (size_t)n * kMlpIn;
-    FE_HIP(hipMemcpyAsync(dx, x, sizeof(float) * kMlpIn * n, hipMemcpyHostToDevice, c->stream), "input upload");
+    std::vector<float> xt((size_t)n * kMlpIn);   // feature-major, as the inference kernel reads
+    for (uint32_t b = 0; b < n; b++)
+        for (int k = 0; k < kMlpIn; k++) xt[(size_t)k * n + b] = x[(size_t)b * kMlpIn + k];
+    FE_HIP(hipMemcpyAsync(dx, xt.data(), sizeof(float) * xt.size(), hipMemcpyHostToDevice, c->stream),
+           "input upload");
     FE_HIP(fe::learn_forward_raw(c, net, dx, n, dy), "mlp forward");
     FE_HIP(hipMemcpyAsync(y, dy, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream), "output download");
     FE_HIP(hipStreamSynchronize(c->stream), "forward wait");
@@ -362,24 +390,31 @@ fe_status fe_learn_features(fe_ctx* c, int net, float* features, float* prev_pre
     FE_CHECK_CTX(c);
     if (check_net(net) != FE_OK) return FE_ERR_INVALID_ARGUMENT;
     if (!features) return fail(FE_ERR_INVALID_ARGUMENT, "null features");
-    DeviceGuard g(c->device);
-    fe_status s = ensure_alloc(c);
+    {
+        DeviceGuard g(c->device);
+        fe_status s = ensure_alloc(c);
+        if (s != FE_OK) return s;
+    }
+    c->ws.force_features = true;
+    fe_status s = net == FE_NET_DENSITY ? fe_density_source(c) : fe_divergence_source(c);
+    c->ws.force_features = false;
     if (s != FE_OK) return s;
-    if (!c->list_valid)
-        return fail(FE_ERR_INVALID_ARGUMENT, "features need the neighbour list of the current positions (fe_density_alpha)");
-    FE_HIP(fe::learn_features(c, net), "warm-start features");
+    DeviceGuard g(c->device);
     const size_t n = c->n;
-    std::vector<float> f(n * kMlpIn);
+    std::vector<float> f(n * kMlpIn), pp(n);
     std::vector<uint32_t> sched(n);
     FE_HIP(hipMemcpyAsync(f.data(), c->ws.feat, sizeof(float) * f.size(), hipMemcpyDeviceToHost, c->stream),
            "features download");
+    FE_HIP(hipMemcpyAsync(pp.data(), c->ws.pprev, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream),
+           "previous pressure download");
     FE_HIP(hipMemcpyAsync(sched.data(), c->sched[0], sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream),
            "schedule download");
-    if (prev_pressure)
-        FE_HIP(hipMemcpyAsync(prev_pressure, c->ws.pprev, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream),
-               "previous pressure download");
     FE_HIP(hipStreamSynchronize(c->stream), "features wait");
-    for (size_t t = 0; t < n; t++) std::memcpy(features + (size_t)sched[t] * kMlpIn, &f[t * kMlpIn], sizeof(float) * kMlpIn);
+    for (size_t t = 0; t < n; t++) {
+        const size_t i = sched[t];
+        for (int k = 0; k < kMlpIn; k++) features[i * kMlpIn + k] = f[(size_t)k * n + t];
+        if (prev_pressure) prev_pressure[i] = pp[t];
+    }
     return FE_OK;
 }
 
@@ -387,10 +422,11 @@ fe_status fe_learn_predict(fe_ctx* c, int net) {
     FE_CHECK_CTX(c);
     if (check_net(net) != FE_OK) return FE_ERR_INVALID_ARGUMENT;
     if (!fe::learn_ready(c, net)) return fail(FE_ERR_INVALID_ARGUMENT, "network not trained or set");
-    if (!c->list_valid)
-        return fail(FE_ERR_INVALID_ARGUMENT, "prediction needs the neighbour list of the current positions (fe_density_alpha)");
+    c->ws.force_features = true;
+    fe_status s = net == FE_NET_DENSITY ? fe_density_source(c) : fe_divergence_source(c);
+    c->ws.force_features = false;
+    if (s != FE_OK) return s;
     DeviceGuard g(c->device);
-    FE_HIP(fe::learn_features(c, net), "warm-start features");
     FE_HIP(fe::learn_infer(c, net), "warm-start inference");
     return FE_OK;
 }

@@ -260,10 +260,14 @@ fe_status fe_learn_forward(fe_ctx* ctx, int net, const float* x, uint32_t n, flo
  * training step computes it (FE_MLP_PARAMS floats), and L * n (optional). */
 fe_status fe_learn_gradient(fe_ctx* ctx, int net, const float* x, const float* t, uint32_t n, float* grad,
                             float* sum_sq);
-/* Features of the current state for solve `net` (n x 12, hash order) and the previous
- * pressures (n, optional). Needs a current neighbour list (after fe_density_alpha). */
+/* Runs the source-term stage of solve `net` (exactly fe_density_source or
+ * fe_divergence_source, with all their effects), which gathers the features in the same
+ * pass, and returns them (n x 12, hash order) with the previous pressures (n, optional).
+ * In fe_step the source-term stages gather the features whenever a learned solve or sample
+ * collection needs them. */
 fe_status fe_learn_features(fe_ctx* ctx, int net, float* features, float* prev_pressure);
-/* The warm start alone: features and prediction into pressures, as a learned solve begins. */
+/* The source-term stage of solve `net` followed by the warm start alone: the prediction
+ * is written to the pressures as a learned solve begins. */
 fe_status fe_learn_predict(fe_ctx* ctx, int net);
 
 #ifdef __cplusplus

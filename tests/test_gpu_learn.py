@@ -130,19 +130,23 @@ def test_training_step_matches_oracle_adam(mlp):
         sim.close()
 
 
-def stages_to(x, point):
-    """Run the reference stages of one frame on a Simulation or an Oracle up to `point`."""
+def stages_to(x, point, source=True):
+    """Run the reference stages of one frame on a Simulation or an Oracle up to the source
+    term of the solve `point` ("density" or "divergence"), that stage included if `source`."""
     x.neighbor_search()
     x.density_alpha()
     x.viscosity()
-    x.density_source()
     if point == "density":
+        if source:
+            x.density_source()
         return
+    x.density_source()
     x.solve_density()
     x.pressure_integration()
     x.neighbor_search()
     x.density_alpha()
-    x.divergence_source()
+    if source:
+        x.divergence_source()
 
 
 def finish_frame(x, point):
@@ -182,12 +186,13 @@ def test_features_and_previous_pressure_match_oracle(oracle_mod, net, point):
         sim.stats()
         stages_to(orc, "divergence")
         finish_frame(orc, "divergence")
-        # the next frame up to the solve under test
-        stages_to(sim, point)
-        stages_to(orc, point)
-        for name in ("densities", "factors", "source_terms", "velocity_a", "position_b"):
-            assert np.array_equal(sim.download(name), orc.download(name)), name
+        # the next frame up to the solve under test; on the GPU the source-term stage is the
+        # one that gathers the features
+        stages_to(sim, point, source=False)
         f, pp = sim.learn_features(net)
+        stages_to(orc, point)
+        for name in ("densities", "factors", "source_terms", "velocity_a", "position_b", "pressures"):
+            assert np.array_equal(sim.download(name), orc.download(name)), name
         vals = sim.download("grid_entries")[: w.n, 1]
         want_pp = (p_dens if net == fe.FE_NET_DENSITY else p_div)[vals]
         assert np.array_equal(pp, want_pp)
@@ -209,7 +214,7 @@ def test_prediction_matches_oracle_forward(mlp):
     try:
         sim.learn_collect(0)
         sim.run(3)
-        stages_to(sim, "density")
+        stages_to(sim, "density", source=False)
         f, _ = sim.learn_features(fe.FE_NET_DENSITY)
         fd = f.astype(np.float64)
         norm = np.concatenate([fd.mean(0), np.maximum(fd.std(0), 1e-6), [150.0, 400.0]]).astype(F)
