@@ -41,6 +41,14 @@ ALGO_BYTES = {
     "divergence_integration": 48.0,
     "stats": 20.0,
 }
+# The source terms with the warm-start features gathered in the same pass also write the 12
+# features (48 B) and read the previous pressure (4 B).
+ALGO_BYTES["density_source_feat"] = ALGO_BYTES["density_source"] + 52.0
+ALGO_BYTES["divergence_source_feat"] = ALGO_BYTES["divergence_source"] + 52.0
+# Warm-start inference: 2 (12 x 64 + 2 x 64 x 64 + 64) multiply-adds per particle on the matrix
+# cores (BF16 dense peak 2.5 PFLOP/s).
+MLP_FLOP = 2.0 * (12 * 64 + 2 * 64 * 64 + 64)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
 def parse():
@@ -53,6 +61,9 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
+    ap.add_argument("--learned-collect", type=int, default=100, help="frames sampled before training")
+    ap.add_argument("--learned-eval", type=int, default=400, help="frames compared after training")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -185,7 +196,72 @@ def main():
         "cpu_baseline": cpu,
     }
     sim.close()
+    if not args.no_learned and args.solver == "converge":
+        out["learned_warm_start"] = learned_comparison(fe, w, opts, args.learned_collect, args.learned_eval)
     print(json.dumps(out), flush=True)
+
+
+def learned_comparison(fe, w, opts, collect, evaluate):
+    """Learned warm start on the same workload: two contexts from the same initial state; the
+    second samples 2048 (features, final pressure) pairs per solve over the first `collect`
+    frames, both networks are trained on the GPU (2000 Adam steps, batch 4096), and then both
+    contexts run `evaluate` more frames, one from the reference start, one from the learned
+    start, to the same 0.1 % / 1 % thresholds. Reports sweeps per frame and frames per second
+    of each over the evaluated frames."""
+    ref = fe.Simulation(w.params, w.positions, device=0, options=opts)
+    lrn = fe.Simulation(w.params, w.positions, device=0, options=opts)
+    lrn.learn_collect(2048, 1 << 18)
+    ref.run(collect)
+    lrn.run(collect)
+    lrn.sync()
+    lrn.learn_collect(0)
+    t0 = time.perf_counter()
+    l0 = lrn.learn_train(fe.FE_NET_DENSITY, steps=2000, batch=4096, lr=2e-3, seed=1)
+    l1 = lrn.learn_train(fe.FE_NET_DIVERGENCE, steps=2000, batch=4096, lr=2e-3, seed=2)
+    train_s = time.perf_counter() - t0
+    lo = fe.FeOptions.from_buffer_copy(opts)
+    lo.warm_start = fe.FE_WARM_START_LEARNED
+    lrn.set_options(lo)
+    res = {}
+    for name, sim in (("reference", ref), ("learned", lrn)):
+        sim.reset_iteration_totals()
+        sim.timer_start()
+        sim.run(evaluate)
+        ms = sim.timer_stop()
+        d, v, f = sim.iteration_totals()
+        res[name] = {"steps_per_s": round(evaluate / (ms / 1e3), 3),
+                     "sweeps_per_step": {"density": round(d / f, 3), "divergence": round(v / f, 3)},
+                     "mean_abs_density_error": round(float(sim.read_stats()[1]) / w.n, 3)}
+    # inference kernel, event-timed over a few learned frames
+    lrn.profile(True)
+    lrn.profile_reset()
+    lrn.run(5)
+    prof = lrn.profile_results()
+    lrn.profile(False)
+    ref.close()
+    lrn.close()
+    rs, ls = res["reference"]["sweeps_per_step"], res["learned"]["sweeps_per_step"]
+    out = {
+        "workload": f"{w.name}: frames {collect}-{collect + evaluate - 1} after sampling frames 0-{collect - 1}",
+        "samples_per_network": collect * 2048,
+        "training": {"steps": 2000, "batch": 4096, "seconds_both_networks": round(train_s, 3),
+                     "density_loss_first_last": [round(float(l0[:20].mean()), 4), round(float(l0[-50:].mean()), 4)],
+                     "divergence_loss_first_last": [round(float(l1[:20].mean()), 4),
+                                                    round(float(l1[-50:].mean()), 4)]},
+        "reference": res["reference"],
+        "learned": res["learned"],
+        "sweep_reduction": round((rs["density"] + rs["divergence"]) / (ls["density"] + ls["divergence"]), 3),
+        "speedup": round(res["learned"]["steps_per_s"] / res["reference"]["steps_per_s"], 3),
+    }
+    if "ws_infer" in prof:
+        launches, total_ms = prof["ws_infer"]
+        avg_s = total_ms / launches / 1e3
+        tf = MLP_FLOP * w.n / avg_s / 1e12
+        out["inference_roofline"] = {"bound": "mfma", "kernel": "ws_infer", "achieved": round(tf, 2),
+                                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                     "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4),
+                                     "avg_launch_us": round(avg_s * 1e6, 2)}
+    return out
 
 
 def cpu_baseline(w, opts, d_avg, v_avg, budget_s):

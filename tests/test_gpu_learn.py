@@ -293,3 +293,37 @@ def test_learned_warm_start_reduces_sweeps():
     finally:
         ref.close()
         lrn.close()
+
+
+def test_learned_warm_start_halves_sweeps_1m():
+    """The 1,048,576-particle dam break: samples from frames 0-99 (2048 per solve, 204,800 per
+    network), both networks trained on the GPU, then frames 100-499 with the learned start
+    against a second context with the reference start. The total Jacobi sweeps to the same
+    0.1 % / 1 % thresholds must drop at least 2x."""
+    w = scene.dam_break_1m(seed=0)
+    ref = fe.Simulation(w.params, w.positions, device=0, options=conv_opts())
+    lrn = fe.Simulation(w.params, w.positions, device=0, options=conv_opts())
+    try:
+        lrn.learn_collect(2048, 1 << 18)
+        ref.run(100)
+        lrn.run(100)
+        lrn.learn_collect(0)
+        lrn.learn_train(fe.FE_NET_DENSITY, steps=2000, batch=4096, lr=2e-3, seed=1)
+        lrn.learn_train(fe.FE_NET_DIVERGENCE, steps=2000, batch=4096, lr=2e-3, seed=2)
+        lrn.set_options(conv_opts(fe.FE_WARM_START_LEARNED))
+        ref.reset_iteration_totals()
+        lrn.reset_iteration_totals()
+        ref.run(400)
+        lrn.run(400)
+        rd, rv, _ = ref.iteration_totals()
+        ld, lv, _ = lrn.iteration_totals()
+        gain = (rd + rv) / max(ld + lv, 1)
+        print(f"1M, frames 100-499: reference {rd} + {rv} sweeps, learned {ld} + {lv}, {gain:.2f}x fewer")
+        assert gain >= 2.0
+        err_l = lrn.read_stats()[1] / w.n
+        err_r = ref.read_stats()[1] / w.n
+        assert err_l <= 1.5 * err_r + 5.0
+        assert np.isfinite(lrn.download("velocity_a")).all()
+    finally:
+        ref.close()
+        lrn.close()
