@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "fe_abi.h"
+#include "fe_device.h"
 
 static_assert(sizeof(fe_params) == 112, "fe_params must match SimulationParams (112 bytes)");
 static_assert(offsetof(fe_params, dt) == 24, "dt offset");
@@ -66,6 +67,10 @@ void free_ctx(fe_ctx* c) {
         if (p) (void)hipFree(p);
     fe::sort_scratch_free(&c->sort);
     fe::learn_free(c);
+    for (void* p : {(void*)c->scene_tab, (void*)c->sid, (void*)c->scene_stats})
+        if (p) (void)hipFree(p);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->host_stats) (void)hipHostFree(c->host_stats);
     if (c->host_resid) (void)hipHostFree(c->host_resid);
     for (auto& r : c->pending) {
@@ -117,6 +122,29 @@ fe_status ensure_staging(fe_ctx* c) {
     return FE_OK;
 }
 
+// The scene table of a batch: per-scene constants and converge thresholds (tol x N_s x 2^32).
+fe_status upload_scene_tab(fe_ctx* c) {
+    if (c->nscenes <= 1) return FE_OK;
+    std::vector<fe::Sph> tab(c->nscenes);
+    for (uint32_t k = 0; k < c->nscenes; k++) {
+        tab[k] = fe::sph_from(c->scene_params[k], c->scene_n[k], c->scene_T[k], c->scene_tab0[k], c->scene_part0[k]);
+        tab[k].thr[0] = (unsigned long long)((double)c->options.density_tol * (double)c->scene_n[k] * 4294967296.0);
+        tab[k].thr[1] = (unsigned long long)((double)c->options.divergence_tol * (double)c->scene_n[k] * 4294967296.0);
+    }
+    FE_HIP(hipStreamSynchronize(c->stream), "scene table wait");
+    FE_HIP(hipMemcpy(c->scene_tab, tab.data(), sizeof(fe::Sph) * tab.size(), hipMemcpyHostToDevice), "scene table");
+    return FE_OK;
+}
+
+// A captured frame holds the kernel arguments of its capture; anything that changes them
+// drops it.
+void drop_graph(fe_ctx* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+}
+
 // Frame control of Renderer::step (src/renderer/mod.rs:231-267).
 void frame_control(const uint32_t stats[3], uint32_t n, fe_params* p, const fe_options* o, uint32_t* dit,
                    uint32_t* vit) {
@@ -156,25 +184,35 @@ fe_status solve(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t* 
 fe_status jacobi(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t* iterations) {
     if (c->options.solver_mode != FE_SOLVER_CONVERGE) {
         for (uint32_t k = 0; k < fixed_iters; k++) {
-            FE_HIP(fe::launch_pressure_force(c, nullptr, 0), "pressure_force");
-            FE_HIP(fe::launch_pressure_update(c, nullptr, nullptr, 0), "pressure_update");
+            FE_HIP(fe::launch_pressure_force(c, nullptr, 0, 0), "pressure_force");
+            FE_HIP(fe::launch_pressure_update(c, nullptr, nullptr, 0, 0), "pressure_update");
         }
         c->last_iters[kind] = fixed_iters;
         c->total_iters[kind] += fixed_iters;
+        for (uint32_t sc = 0; c->nscenes > 1 && sc < c->nscenes; sc++) {
+            c->scene_iters[kind][sc] = fixed_iters;
+            c->scene_total_iters[kind][sc] += fixed_iters;
+        }
         if (iterations) *iterations = fixed_iters;
         return FE_OK;
     }
     // FE_SOLVER_CONVERGE: sweep k writes its residual to slot[k]; sweep k+1 skips itself on
     // the device once slot[k] <= thr. The host launches chunks of sweeps ahead and stops
-    // launching once a finished chunk shows convergence.
-    unsigned long long* slot = c->resid + (size_t)kind * fe::kMaxSolveIters;
-    unsigned long long* hslot = c->host_resid + (size_t)kind * fe::kMaxSolveIters;
+    // launching once a finished chunk shows convergence. A batch keeps one row of slots per
+    // scene (kMaxSolveIters apart) and a threshold per scene; each scene's particles skip on
+    // their own, and the host stops once every scene has converged.
+    const uint32_t S = c->nscenes;
+    unsigned long long* slot = c->resid + (size_t)kind * S * fe::kMaxSolveIters;
+    unsigned long long* hslot = c->host_resid + (size_t)kind * S * fe::kMaxSolveIters;
     uint32_t maxit = c->options.max_iterations;
     if (maxit < 1) maxit = 1;
     if (maxit > fe::kMaxSolveIters) maxit = fe::kMaxSolveIters;
     const unsigned long long thr =
         (unsigned long long)((double)tol * (double)c->n * 4294967296.0);
-    FE_HIP(fe::launch_fill_u64(c, slot, maxit), "fill residuals");
+    std::vector<unsigned long long> thr_s(S, thr);
+    for (uint32_t k = 0; S > 1 && k < S; k++)
+        thr_s[k] = (unsigned long long)((double)tol * (double)c->scene_n[k] * 4294967296.0);
+    FE_HIP(fe::launch_fill_u64(c, slot, S > 1 ? S * fe::kMaxSolveIters : maxit), "fill residuals");
     // Chunk sizes: the previous frame's sweep count first (the usual answer), then 2, 4, 8,
     // 16, ... Chunk j's residuals are read back while chunk j+1 runs.
     struct Chunk {
@@ -182,25 +220,41 @@ fe_status jacobi(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t*
         uint32_t beg, end;
     };
     std::vector<Chunk> chunks;
+    std::vector<uint32_t> found(S, 0);
+    uint32_t pending = S;
+    auto scan = [&](const Chunk& ch) {
+        for (uint32_t sc = 0; sc < S; sc++) {
+            if (found[sc]) continue;
+            for (uint32_t q = ch.beg; q < ch.end; q++)
+                if (hslot[(size_t)sc * fe::kMaxSolveIters + q] <= thr_s[sc]) {
+                    found[sc] = q + 1;
+                    pending--;
+                    break;
+                }
+        }
+    };
     uint32_t first = c->last_iters[kind] ? c->last_iters[kind] : 4;
     if (first > maxit) first = maxit;
     uint32_t chunk_beg = 0, chunk_end = first, grow = 2;
-    uint32_t launched = 0, found = 0;
+    uint32_t launched = 0;
     size_t checked_chunks = 0;
-    bool done = false;
     fe_status st = FE_OK;
-    for (uint32_t k = 0; k < maxit && !done; k++) {
+    for (uint32_t k = 0; k < maxit && pending; k++) {
         const unsigned long long* prev = k == 0 ? nullptr : slot + (k - 1);
-        hipError_t e = fe::launch_pressure_force(c, prev, thr);
-        if (e == hipSuccess) e = fe::launch_pressure_update(c, slot + k, prev, thr);
+        hipError_t e = fe::launch_pressure_force(c, prev, thr, (uint32_t)kind);
+        if (e == hipSuccess) e = fe::launch_pressure_update(c, slot + k, prev, thr, (uint32_t)kind);
         if (e != hipSuccess) { st = hip_fail(e, "jacobi sweep"); break; }
         launched = k + 1;
         if (launched < chunk_end && launched < maxit) continue;
         hipEvent_t ev;
         e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(hslot + chunk_beg, slot + chunk_beg,
-                               sizeof(unsigned long long) * (launched - chunk_beg), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) {
+            const size_t w = sizeof(unsigned long long) * (launched - chunk_beg);
+            const size_t pitch = sizeof(unsigned long long) * fe::kMaxSolveIters;
+            e = S == 1 ? hipMemcpyAsync(hslot + chunk_beg, slot + chunk_beg, w, hipMemcpyDeviceToHost, c->stream)
+                       : hipMemcpy2DAsync(hslot + chunk_beg, pitch, slot + chunk_beg, pitch, w, S,
+                                          hipMemcpyDeviceToHost, c->stream);
+        }
         if (e == hipSuccess) e = hipEventRecord(ev, c->stream);
         if (e != hipSuccess) { st = hip_fail(e, "residual readback"); break; }
         chunks.push_back(Chunk{ev, chunk_beg, launched});
@@ -212,21 +266,26 @@ fe_status jacobi(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t*
             const Chunk& ch = chunks[chunks.size() - 2];
             e = hipEventSynchronize(ch.ev);
             if (e != hipSuccess) { st = hip_fail(e, "residual wait"); break; }
-            for (uint32_t q = ch.beg; q < ch.end; q++)
-                if (hslot[q] <= thr) { found = q + 1; done = true; break; }
+            scan(ch);
             checked_chunks = chunks.size() - 1;
         }
     }
     if (st == FE_OK && !chunks.empty()) {
         hipError_t e = hipEventSynchronize(chunks.back().ev);
         if (e != hipSuccess) st = hip_fail(e, "residual wait");
-        for (size_t j = checked_chunks; st == FE_OK && !found && j < chunks.size(); j++)
-            for (uint32_t q = chunks[j].beg; q < chunks[j].end; q++)
-                if (hslot[q] <= thr) { found = q + 1; break; }
+        for (size_t j = checked_chunks; st == FE_OK && pending && j < chunks.size(); j++) scan(chunks[j]);
     }
     for (const Chunk& ch : chunks) (void)hipEventDestroy(ch.ev);
     if (st != FE_OK) return st;
-    const uint32_t its = found ? found : launched;
+    uint32_t its = 0;
+    for (uint32_t sc = 0; sc < S; sc++) {
+        const uint32_t it = found[sc] ? found[sc] : launched;
+        its = it > its ? it : its;
+        if (S > 1) {
+            c->scene_iters[kind][sc] = it;
+            c->scene_total_iters[kind][sc] += it;
+        }
+    }
     c->last_iters[kind] = its;
     c->total_iters[kind] += its;
     if (iterations) *iterations = its;
@@ -248,7 +307,8 @@ fe_status neighbor_search(fe_ctx* c) {
     c->ws.feat_fresh = false;
     // gather schedule: particles in Morton order of their cells
     FE_HIP(fe::launch_morton_keys(c), "morton keys");
-    FE_HIP(fe::radix_sort_pairs(c, c->mkeys, c->sched, c->n, c->morton_bits, "schedule"), "schedule sort");
+    FE_HIP(fe::radix_sort_pairs(c, c->mkeys, c->sched, c->n, c->morton_bits + c->scene_bits, "schedule"),
+           "schedule sort");
     FE_HIP(fe::launch_schedule_mirrors(c), "schedule mirrors");
     c->ns_fresh = true;
     c->da_fresh = false;
@@ -300,11 +360,53 @@ fe_status step(fe_ctx* c, float max_dt) {
         if (!(c->params.dt > 0.0f)) break;   // a non-positive dt would never end the frame
     }
     FE_HIP(fe::launch_stats(c), "stats");
-    FE_HIP(hipMemcpyAsync(c->host_stats, c->stats, 12, hipMemcpyDeviceToHost, c->stream), "stats copy");
-    FE_HIP(hipEventRecord(c->stats_event, c->stream), "stats event");
-    c->stats_pending = true;
+    if (c->nscenes == 1) {
+        FE_HIP(hipMemcpyAsync(c->host_stats, c->stats, 12, hipMemcpyDeviceToHost, c->stream), "stats copy");
+        FE_HIP(hipEventRecord(c->stats_event, c->stream), "stats event");
+        c->stats_pending = true;
+    }
     c->frames++;
     return FE_OK;
+}
+
+// A batch frame in FIXED mode from a captured graph: the steady-state frame (frame-start
+// search reused) is captured once and replayed; the first frame after creation, an upload or
+// a parameter change runs eagerly.
+fe_status step_graph(fe_ctx* c, float max_dt) {
+    if (!(c->ns_fresh && c->da_fresh)) return step(c, max_dt);
+    if (!c->graph_exec) {
+        FE_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        fe_status st = step(c, max_dt);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (st != FE_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return hip_fail(e, "graph capture");
+        c->graph = g;
+        e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            drop_graph(c);
+            return hip_fail(e, "graph instantiate");
+        }
+        // the capture recorded the frame; it has not run yet
+        c->frames--;
+        for (int k = 0; k < 2; k++) c->total_iters[k] -= c->last_iters[k];
+        for (int k = 0; k < 2; k++)
+            for (uint32_t sc = 0; sc < c->nscenes; sc++) c->scene_total_iters[k][sc] -= c->scene_iters[k][sc];
+    }
+    FE_HIP(hipGraphLaunch(c->graph_exec, c->stream), "graph launch");
+    c->frames++;
+    for (int k = 0; k < 2; k++) c->total_iters[k] += c->last_iters[k];
+    for (int k = 0; k < 2; k++)
+        for (uint32_t sc = 0; sc < c->nscenes; sc++) c->scene_total_iters[k][sc] += c->scene_iters[k][sc];
+    return FE_OK;
+}
+
+fe_status run_frame(fe_ctx* c, float max_dt) {
+    const bool graph = c->batch_graph && c->nscenes > 1 && c->options.solver_mode == FE_SOLVER_FIXED && !c->profiling;
+    return graph ? step_graph(c, max_dt) : step(c, max_dt);
 }
 
 }  // namespace
@@ -349,6 +451,71 @@ static fe_status resolve_profile(fe_ctx* c) {
     return FE_OK;
 }
 
+// Device buffers of a context whose n, T, log2T and nscenes are set; position_a from xyz
+// (n x 3), everything else zero (GpuPhysicsData::new, particle.rs:164-299).
+static fe_status alloc_ctx(fe_ctx* c, const float* xyz) {
+    const uint32_t n = c->n;
+#define ALLOC(ptr, bytes)                                        \
+    do {                                                         \
+        hipError_t ee = hipMalloc((void**)&(ptr), (bytes));      \
+        if (ee != hipSuccess) return hip_fail(ee, "hipMalloc " #ptr); \
+    } while (0)
+    const size_t v4 = 16 * (size_t)n, s1 = 4 * (size_t)n, tb = 4 * (size_t)c->T;
+    const size_t slots = (size_t)2 * c->nscenes * fe::kMaxSolveIters;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    ALLOC(c->pos_a, v4); ALLOC(c->pos_b, v4); ALLOC(c->vel_a, v4); ALLOC(c->vel_b, v4);
+    ALLOC(c->colors, v4); ALLOC(c->pacc, v4);
+    ALLOC(c->dens, s1); ALLOC(c->fac, s1); ALLOC(c->src, s1); ALLOC(c->pres, s1);
+    ALLOC(c->keys[0], tb); ALLOC(c->keys[1], tb); ALLOC(c->vals[0], tb); ALLOC(c->vals[1], tb);
+    ALLOC(c->range, 2 * tb);
+    ALLOC(c->stats, 16);
+    ALLOC(c->resid, sizeof(unsigned long long) * slots);
+    ALLOC(c->partial, sizeof(unsigned long long) * fe::gather_partials(n));
+    ALLOC(c->stats_partial, sizeof(uint32_t) * 3 * fe::kStatsBlocks);
+    ALLOC(c->nbr, sizeof(uint32_t) * fe::kMaxNbr * (((size_t)n + 15) / 16 * 16));   // whole 16-item tiles
+    ALLOC(c->nbr_count, sizeof(uint32_t) * (size_t)n);
+    for (int k = 0; k < 2; k++) {
+        ALLOC(c->sched[k], sizeof(uint32_t) * (size_t)n);
+        ALLOC(c->mkeys[k], sizeof(uint32_t) * (size_t)n);
+    }
+    ALLOC(c->inv, s1); ALLOC(c->pos_m, v4); ALLOC(c->vb_m, v4); ALLOC(c->va_m, v4);
+    ALLOC(c->dens_m, s1); ALLOC(c->prho_m, s1); ALLOC(c->pacc_m, v4);
+#undef ALLOC
+    e = fe::sort_scratch_alloc(&c->sort, c->T);
+    if (e != hipSuccess) return hip_fail(e, "sort scratch");
+    e = hipHostMalloc((void**)&c->host_stats, 16, hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&c->host_resid, sizeof(unsigned long long) * slots, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->stats_event, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreate(&c->timer_start);
+    if (e == hipSuccess) e = hipEventCreate(&c->timer_stop);
+    if (e != hipSuccess) return hip_fail(e, "host buffers / events");
+
+    // GpuPhysicsData::new: position_a = (xyz, 1); velocities, pressures and stats zero.
+    std::vector<float> p4(4 * (size_t)n);
+    for (size_t i = 0; i < n; i++) {
+        p4[4 * i] = xyz[3 * i];
+        p4[4 * i + 1] = xyz[3 * i + 1];
+        p4[4 * i + 2] = xyz[3 * i + 2];
+        p4[4 * i + 3] = 1.0f;
+    }
+    e = hipMemcpy(c->pos_a, p4.data(), v4, hipMemcpyHostToDevice);
+    for (void* z : {(void*)c->pos_b, (void*)c->vel_a, (void*)c->vel_b, (void*)c->colors, (void*)c->pacc})
+        if (e == hipSuccess) e = hipMemset(z, 0, v4);
+    for (void* z : {(void*)c->dens, (void*)c->fac, (void*)c->src, (void*)c->pres})
+        if (e == hipSuccess) e = hipMemset(z, 0, s1);
+    if (e == hipSuccess) e = hipMemset(c->stats, 0, 16);
+    if (e == hipSuccess) e = hipMemset(c->range, 0xFF, 2 * tb);
+    if (e == hipSuccess) e = hipMemset(c->keys[0], 0xFF, tb);
+    if (e == hipSuccess) e = hipMemset(c->vals[0], 0xFF, tb);
+    if (e == hipSuccess) e = fe::launch_iota(c, c->sched[0], n);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, "initial upload");
+    std::memset(c->host_stats, 0, 16);
+    return FE_OK;
+}
+
 extern "C" {
 
 const char* fe_version(void) { return "fluid_engine-mi355x 0.1 (gfx950)"; }
@@ -387,66 +554,11 @@ fe_status fe_create(const fe_params* params, const float* xyz, uint32_t n, int d
         c->T <<= 1;
         c->log2T++;
     }
-#define ALLOC(ptr, bytes)                                        \
-    do {                                                         \
-        hipError_t ee = hipMalloc((void**)&(ptr), (bytes));      \
-        if (ee != hipSuccess) {                                  \
-            free_ctx(c);                                         \
-            return hip_fail(ee, "hipMalloc " #ptr);              \
-        }                                                        \
-    } while (0)
-    const size_t v4 = 16 * (size_t)n, s1 = 4 * (size_t)n, tb = 4 * (size_t)c->T;
-    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) { free_ctx(c); return hip_fail(e, "hipStreamCreate"); }
-    ALLOC(c->pos_a, v4); ALLOC(c->pos_b, v4); ALLOC(c->vel_a, v4); ALLOC(c->vel_b, v4);
-    ALLOC(c->colors, v4); ALLOC(c->pacc, v4);
-    ALLOC(c->dens, s1); ALLOC(c->fac, s1); ALLOC(c->src, s1); ALLOC(c->pres, s1);
-    ALLOC(c->keys[0], tb); ALLOC(c->keys[1], tb); ALLOC(c->vals[0], tb); ALLOC(c->vals[1], tb);
-    ALLOC(c->range, 2 * tb);
-    ALLOC(c->stats, 16);
-    ALLOC(c->resid, sizeof(unsigned long long) * 2 * fe::kMaxSolveIters);
-    ALLOC(c->partial, sizeof(unsigned long long) * fe::gather_partials(n));
-    ALLOC(c->stats_partial, sizeof(uint32_t) * 3 * fe::kStatsBlocks);
-    ALLOC(c->nbr, sizeof(uint32_t) * fe::kMaxNbr * (((size_t)n + 15) / 16 * 16));   // whole 16-item tiles
-    ALLOC(c->nbr_count, sizeof(uint32_t) * (size_t)n);
-    for (int k = 0; k < 2; k++) {
-        ALLOC(c->sched[k], sizeof(uint32_t) * (size_t)n);
-        ALLOC(c->mkeys[k], sizeof(uint32_t) * (size_t)n);
+    fe_status st = alloc_ctx(c, xyz);
+    if (st != FE_OK) {
+        free_ctx(c);
+        return st;
     }
-    ALLOC(c->inv, s1); ALLOC(c->pos_m, v4); ALLOC(c->vb_m, v4); ALLOC(c->va_m, v4);
-    ALLOC(c->dens_m, s1); ALLOC(c->prho_m, s1); ALLOC(c->pacc_m, v4);
-#undef ALLOC
-    e = fe::sort_scratch_alloc(&c->sort, c->T);
-    if (e != hipSuccess) { free_ctx(c); return hip_fail(e, "sort scratch"); }
-    e = hipHostMalloc((void**)&c->host_stats, 16, hipHostMallocDefault);
-    if (e == hipSuccess)
-        e = hipHostMalloc((void**)&c->host_resid, sizeof(unsigned long long) * 2 * fe::kMaxSolveIters,
-                          hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->stats_event, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreate(&c->timer_start);
-    if (e == hipSuccess) e = hipEventCreate(&c->timer_stop);
-    if (e != hipSuccess) { free_ctx(c); return hip_fail(e, "host buffers / events"); }
-
-    // GpuPhysicsData::new: position_a = (xyz, 1); velocities, pressures and stats zero.
-    std::vector<float> p4(4 * (size_t)n);
-    for (size_t i = 0; i < n; i++) {
-        p4[4 * i] = xyz[3 * i];
-        p4[4 * i + 1] = xyz[3 * i + 1];
-        p4[4 * i + 2] = xyz[3 * i + 2];
-        p4[4 * i + 3] = 1.0f;
-    }
-    e = hipMemcpy(c->pos_a, p4.data(), v4, hipMemcpyHostToDevice);
-    for (void* z : {(void*)c->pos_b, (void*)c->vel_a, (void*)c->vel_b, (void*)c->colors, (void*)c->pacc})
-        if (e == hipSuccess) e = hipMemset(z, 0, v4);
-    for (void* z : {(void*)c->dens, (void*)c->fac, (void*)c->src, (void*)c->pres})
-        if (e == hipSuccess) e = hipMemset(z, 0, s1);
-    if (e == hipSuccess) e = hipMemset(c->stats, 0, 16);
-    if (e == hipSuccess) e = hipMemset(c->range, 0xFF, 2 * tb);
-    if (e == hipSuccess) e = hipMemset(c->keys[0], 0xFF, tb);
-    if (e == hipSuccess) e = hipMemset(c->vals[0], 0xFF, tb);
-    if (e == hipSuccess) e = fe::launch_iota(c, c->sched[0], n);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) { free_ctx(c); return hip_fail(e, "initial upload"); }
     std::memset(c->host_stats, 0, 16);
     *out = c;
     return FE_OK;
@@ -464,6 +576,8 @@ fe_status fe_set_params(fe_ctx* c, const fe_params* p) {
     FE_CHECK_CTX(c);
     if (!p) return fail(FE_ERR_INVALID_ARGUMENT, "null params");
     if (!(p->smoothing_radius > 0.0f)) return fail(FE_ERR_INVALID_ARGUMENT, "smoothing_radius must be > 0");
+    if (c->nscenes > 1) return fail(FE_ERR_UNSUPPORTED, "a batch takes per-scene parameters (fe_batch_set_params)");
+    drop_graph(c);
     c->params = *p;
     c->morton_bits = morton_bits_for(*p);
     return FE_OK;
@@ -490,8 +604,13 @@ fe_status fe_set_options(fe_ctx* c, const fe_options* o) {
                     "warm_start LEARNED needs a network: fe_learn_train or fe_learn_set_weights first");
     if (o->solver_mode == FE_SOLVER_CONVERGE && (o->max_iterations < 1 || o->max_iterations > fe::kMaxSolveIters))
         return fail(FE_ERR_INVALID_ARGUMENT, "max_iterations must be in [1, 1024]");
+    if (c->nscenes > 1 && (o->sort_algorithm != FE_SORT_RADIX || o->use_cfl || o->use_solver_error_threshold ||
+                           o->warm_start != FE_WARM_START_REFERENCE))
+        return fail(FE_ERR_UNSUPPORTED,
+                    "a batch runs the radix sort, fixed dt, no early-exit flags and the reference warm start");
+    drop_graph(c);
     c->options = *o;
-    return FE_OK;
+    return upload_scene_tab(c);
 }
 
 fe_status fe_get_options(const fe_ctx* c, fe_options* out) {
@@ -522,6 +641,9 @@ fe_status fe_upload(fe_ctx* c, int field, const void* src, size_t bytes) {
     c->ns_fresh = false;
     c->da_fresh = false;
     fe::learn_invalidate(c);
+    drop_graph(c);
+    if (c->nscenes > 1 && (field == FE_FIELD_GRID_ENTRIES || field == FE_FIELD_GRID_START))
+        return fail(FE_ERR_UNSUPPORTED, "a batch's grid fields are per scene (fe_batch_download)");
     // schedule-ordered mirrors of the uploaded field are stale now
     switch (field) {
         case FE_FIELD_POSITION_B: c->mirrors &= ~fe::kMirrorPos; break;
@@ -551,6 +673,8 @@ fe_status fe_upload(fe_ctx* c, int field, const void* src, size_t bytes) {
 
 fe_status fe_download(fe_ctx* c, int field, void* dst, size_t bytes) {
     FE_CHECK_CTX(c);
+    if (c->nscenes > 1 && (field == FE_FIELD_GRID_ENTRIES || field == FE_FIELD_GRID_START))
+        return fail(FE_ERR_UNSUPPORTED, "a batch's grid fields are per scene (fe_batch_download)");
     size_t want;
     void* dev;
     if (!field_info(c, field, &want, &dev)) return fail(FE_ERR_INVALID_ARGUMENT, "unknown field");
@@ -598,6 +722,8 @@ FE_STAGE(fe_neighbor_search, {
     if (s != FE_OK) return s;
 })
 FE_STAGE(fe_sort, {
+    if (c->nscenes > 1) return fail(FE_ERR_UNSUPPORTED, "the stand-alone sort stage takes one scene");
+    drop_graph(c);
     if (c->options.sort_algorithm == FE_SORT_BITONIC)
         FE_HIP(fe::bitonic_sort(c), "bitonic sort");
     else
@@ -626,12 +752,12 @@ FE_STAGE(fe_density_source, {
 FE_STAGE(fe_pressure_force, {
     fe_status s = ensure_ends(c);
     if (s != FE_OK) return s;
-    FE_HIP(fe::launch_pressure_force(c, nullptr, 0), "pressure_force");
+    FE_HIP(fe::launch_pressure_force(c, nullptr, 0, 0), "pressure_force");
 })
 FE_STAGE(fe_pressure_update, {
     fe_status s = ensure_ends(c);
     if (s != FE_OK) return s;
-    FE_HIP(fe::launch_pressure_update(c, nullptr, nullptr, 0), "pressure_update");
+    FE_HIP(fe::launch_pressure_update(c, nullptr, nullptr, 0, 0), "pressure_update");
 })
 FE_STAGE(fe_pressure_integration, { FE_HIP(fe::launch_pressure_integration(c), "pressure_integration"); })
 FE_STAGE(fe_divergence_source, {
@@ -657,14 +783,14 @@ fe_status fe_solve_divergence(fe_ctx* c, uint32_t* iterations) {
 fe_status fe_step(fe_ctx* c, float max_dt) {
     FE_CHECK_CTX(c);
     DeviceGuard g(c->device);
-    return step(c, max_dt);
+    return run_frame(c, max_dt);
 }
 
 fe_status fe_run(fe_ctx* c, float max_dt, uint32_t count) {
     FE_CHECK_CTX(c);
     DeviceGuard g(c->device);
     for (uint32_t k = 0; k < count; k++) {
-        fe_status s = step(c, max_dt);
+        fe_status s = run_frame(c, max_dt);
         if (s != FE_OK) return s;
     }
     return FE_OK;
@@ -778,6 +904,227 @@ fe_status fe_frame_control(const uint32_t stats[3], uint32_t n, const fe_params*
     if (dt_out) *dt_out = p.dt;
     if (density_iters) *density_iters = dit;
     if (divergence_iters) *divergence_iters = vit;
+    return FE_OK;
+}
+
+/* ---- batched scenes ----------------------------------------------------------------------- */
+
+fe_status fe_batch_create(const fe_params* params, const float* const* xyz, const uint32_t* counts, uint32_t scenes,
+                          int device, fe_ctx** out) {
+    if (!params || !xyz || !counts || !out) return fail(FE_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (scenes == 0 || scenes > 256) return fail(FE_ERR_INVALID_ARGUMENT, "scenes must be in [1, 256]");
+    uint64_t n_all = 0, t_all = 0;
+    for (uint32_t k = 0; k < scenes; k++) {
+        if (counts[k] == 0 || counts[k] > (1u << 28) || !xyz[k])
+            return fail(FE_ERR_INVALID_ARGUMENT, "scene " + std::to_string(k) + ": bad particle count or positions");
+        if (!(params[k].smoothing_radius > 0.0f))
+            return fail(FE_ERR_INVALID_ARGUMENT, "scene " + std::to_string(k) + ": smoothing_radius must be > 0");
+        if (std::memcmp(&params[k].dt, &params[0].dt, sizeof(float)) != 0 ||
+            params[k].density_solver_iterations != params[0].density_solver_iterations ||
+            params[k].divergence_solver_iterations != params[0].divergence_solver_iterations)
+            return fail(FE_ERR_INVALID_ARGUMENT, "scenes advance in lockstep: dt and sweep counts must be equal");
+        uint64_t T = 1;
+        while (T < counts[k]) T <<= 1;
+        n_all += counts[k];
+        t_all += T;
+    }
+    if (n_all > (1u << 30) || t_all > (1u << 31)) return fail(FE_ERR_INVALID_ARGUMENT, "batch too large");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FE_ERR_DEVICE, "no HIP device available");
+    if (device < 0 && hipGetDevice(&device) != hipSuccess) device = 0;
+    if (device >= ndev) return fail(FE_ERR_INVALID_ARGUMENT, "device index out of range");
+    DeviceGuard guard(device);
+
+    fe_ctx* c = new fe_ctx();
+    c->device = device;
+    c->params = params[0];
+    default_options(&c->options);
+    c->nscenes = scenes;
+    c->n = (uint32_t)n_all;
+    c->T = (uint32_t)t_all;
+    c->log2T = 0;
+    while ((1ull << c->log2T) < t_all) c->log2T++;
+    uint32_t sbits = 0;
+    while ((1u << sbits) < scenes) sbits++;
+    uint32_t axis = 1;
+    for (uint32_t k = 0; k < scenes; k++) axis = std::max(axis, morton_bits_for(params[k]) / 3);
+    axis = std::min(axis, (32u - sbits) / 3u);
+    c->morton_bits = 3 * axis;
+    c->scene_bits = sbits;
+    c->scene_params.assign(params, params + scenes);
+    std::vector<float> xyz_all(3 * n_all);
+    std::vector<uint8_t> sid(n_all);
+    uint32_t part = 0, tab = 0;
+    for (uint32_t k = 0; k < scenes; k++) {
+        uint32_t T = 1;
+        while (T < counts[k]) T <<= 1;
+        c->scene_n.push_back(counts[k]);
+        c->scene_T.push_back(T);
+        c->scene_part0.push_back(part);
+        c->scene_tab0.push_back(tab);
+        std::memcpy(&xyz_all[3 * (size_t)part], xyz[k], sizeof(float) * 3 * (size_t)counts[k]);
+        std::memset(&sid[part], (int)k, counts[k]);
+        part += counts[k];
+        tab += T;
+    }
+    for (int k = 0; k < 2; k++) {
+        c->scene_iters[k].assign(scenes, 0);
+        c->scene_total_iters[k].assign(scenes, 0);
+    }
+    fe_status st = alloc_ctx(c, xyz_all.data());
+    hipError_t e = hipSuccess;
+    if (st == FE_OK) {
+        e = hipMalloc((void**)&c->scene_tab, sizeof(fe::Sph) * scenes);
+        if (e == hipSuccess) e = hipMalloc((void**)&c->sid, n_all);
+        if (e == hipSuccess) e = hipMalloc((void**)&c->scene_stats, sizeof(uint32_t) * 3 * scenes);
+        if (e == hipSuccess) e = hipMemcpy(c->sid, sid.data(), n_all, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(c->scene_stats, 0, sizeof(uint32_t) * 3 * scenes);
+        if (e != hipSuccess) st = hip_fail(e, "batch tables");
+    }
+    if (st == FE_OK) st = upload_scene_tab(c);
+    if (st != FE_OK) {
+        free_ctx(c);
+        return st;
+    }
+    *out = c;
+    return FE_OK;
+}
+
+fe_status fe_batch_scenes(const fe_ctx* c, uint32_t* scenes) {
+    FE_CHECK_CTX(c);
+    if (!scenes) return fail(FE_ERR_INVALID_ARGUMENT, "null out");
+    *scenes = c->nscenes;
+    return FE_OK;
+}
+
+static fe_status check_scene(const fe_ctx* c, uint32_t scene) {
+    if (c->nscenes <= 1) return fail(FE_ERR_INVALID_ARGUMENT, "not a batch context (fe_batch_create)");
+    if (scene >= c->nscenes) return fail(FE_ERR_INVALID_ARGUMENT, "scene index out of range");
+    return FE_OK;
+}
+
+fe_status fe_batch_scene_info(const fe_ctx* c, uint32_t scene, uint32_t* n, uint32_t* table_size) {
+    FE_CHECK_CTX(c);
+    fe_status s = check_scene(c, scene);
+    if (s != FE_OK) return s;
+    if (n) *n = c->scene_n[scene];
+    if (table_size) *table_size = c->scene_T[scene];
+    return FE_OK;
+}
+
+fe_status fe_batch_set_params(fe_ctx* c, uint32_t scene, const fe_params* p) {
+    FE_CHECK_CTX(c);
+    fe_status s = check_scene(c, scene);
+    if (s != FE_OK) return s;
+    if (!p || !(p->smoothing_radius > 0.0f)) return fail(FE_ERR_INVALID_ARGUMENT, "bad params");
+    if (std::memcmp(&p->dt, &c->params.dt, sizeof(float)) != 0 ||
+        p->density_solver_iterations != c->params.density_solver_iterations ||
+        p->divergence_solver_iterations != c->params.divergence_solver_iterations)
+        return fail(FE_ERR_INVALID_ARGUMENT, "scenes advance in lockstep: dt and sweep counts must stay equal");
+    DeviceGuard g(c->device);
+    drop_graph(c);
+    c->scene_params[scene] = *p;
+    c->ns_fresh = c->da_fresh = false;   // h may have changed
+    return upload_scene_tab(c);
+}
+
+// One scene's field in its own layout: particle fields are the scene's range; the grid
+// fields are rebuilt as the scene's own T_s-entry table (indices relative to the scene).
+fe_status fe_batch_download(fe_ctx* c, uint32_t scene, int field, void* dst, size_t bytes) {
+    FE_CHECK_CTX(c);
+    fe_status s = check_scene(c, scene);
+    if (s != FE_OK) return s;
+    if (!dst) return fail(FE_ERR_INVALID_ARGUMENT, "null destination");
+    DeviceGuard g(c->device);
+    const uint32_t n = c->scene_n[scene], T = c->scene_T[scene], p0 = c->scene_part0[scene], t0 = c->scene_tab0[scene];
+    if (field == FE_FIELD_GRID_ENTRIES || field == FE_FIELD_GRID_START) {
+        s = ensure_ends(c);
+        if (s != FE_OK) return s;
+        const size_t want = field == FE_FIELD_GRID_ENTRIES ? 8 * (size_t)T : 4 * (size_t)T;
+        if (bytes != want) return fail(FE_ERR_SIZE_MISMATCH, "field size mismatch: expected " + std::to_string(want));
+        uint32_t* o = static_cast<uint32_t*>(dst);
+        if (field == FE_FIELD_GRID_ENTRIES) {
+            std::vector<uint32_t> k(n), v(n);
+            FE_HIP(hipMemcpyAsync(k.data(), c->keys[0] + p0, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream), "download");
+            FE_HIP(hipMemcpyAsync(v.data(), c->vals[0] + p0, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream), "download");
+            FE_HIP(hipStreamSynchronize(c->stream), "download wait");
+            for (uint32_t i = 0; i < T; i++) {
+                o[2 * (size_t)i] = i < n ? k[i] - t0 : fe::kSentinel;
+                o[2 * (size_t)i + 1] = i < n ? v[i] - p0 : fe::kSentinel;
+            }
+        } else {
+            std::vector<uint2> r(T);
+            FE_HIP(hipMemcpyAsync(r.data(), c->range + t0, 8 * (size_t)T, hipMemcpyDeviceToHost, c->stream), "download");
+            FE_HIP(hipStreamSynchronize(c->stream), "download wait");
+            for (uint32_t i = 0; i < T; i++) o[i] = r[i].x == fe::kSentinel ? fe::kSentinel : r[i].x - p0;
+        }
+        return FE_OK;
+    }
+    size_t whole;
+    void* dev;
+    if (!field_info(c, field, &whole, &dev) || !dev) return fail(FE_ERR_INVALID_ARGUMENT, "unknown field");
+    const size_t per = whole / c->n;
+    if (bytes != per * n) return fail(FE_ERR_SIZE_MISMATCH, "field size mismatch: expected " + std::to_string(per * n));
+    FE_HIP(hipMemcpyAsync(dst, static_cast<char*>(dev) + per * p0, bytes, hipMemcpyDeviceToHost, c->stream), "download");
+    FE_HIP(hipStreamSynchronize(c->stream), "download wait");
+    return FE_OK;
+}
+
+fe_status fe_batch_upload(fe_ctx* c, uint32_t scene, int field, const void* src, size_t bytes) {
+    FE_CHECK_CTX(c);
+    fe_status s = check_scene(c, scene);
+    if (s != FE_OK) return s;
+    if (!src) return fail(FE_ERR_INVALID_ARGUMENT, "null source");
+    if (field == FE_FIELD_GRID_ENTRIES || field == FE_FIELD_GRID_START)
+        return fail(FE_ERR_UNSUPPORTED, "a batch's grid fields are rebuilt by the neighbour search");
+    size_t whole;
+    void* dev;
+    if (!field_info(c, field, &whole, &dev) || !dev) return fail(FE_ERR_INVALID_ARGUMENT, "unknown field");
+    const size_t per = whole / c->n;
+    const uint32_t n = c->scene_n[scene], p0 = c->scene_part0[scene];
+    if (bytes != per * n) return fail(FE_ERR_SIZE_MISMATCH, "field size mismatch: expected " + std::to_string(per * n));
+    DeviceGuard g(c->device);
+    drop_graph(c);
+    c->ns_fresh = c->da_fresh = false;
+    c->list_valid = false;
+    c->mirrors = 0;
+    FE_HIP(hipMemcpyAsync(static_cast<char*>(dev) + per * p0, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
+    FE_HIP(hipStreamSynchronize(c->stream), "upload wait");
+    return FE_OK;
+}
+
+fe_status fe_batch_read_stats(fe_ctx* c, uint32_t scene, uint32_t out[3]) {
+    FE_CHECK_CTX(c);
+    fe_status s = check_scene(c, scene);
+    if (s != FE_OK) return s;
+    if (!out) return fail(FE_ERR_INVALID_ARGUMENT, "null out");
+    DeviceGuard g(c->device);
+    FE_HIP(hipMemcpyAsync(c->host_stats, c->scene_stats + 3 * (size_t)scene, 12, hipMemcpyDeviceToHost, c->stream),
+           "stats copy");
+    FE_HIP(hipStreamSynchronize(c->stream), "stats wait");
+    std::memcpy(out, c->host_stats, 12);
+    return FE_OK;
+}
+
+fe_status fe_batch_iterations(fe_ctx* c, uint32_t scene, uint32_t* density, uint32_t* divergence,
+                              uint64_t* density_total, uint64_t* divergence_total) {
+    FE_CHECK_CTX(c);
+    fe_status s = check_scene(c, scene);
+    if (s != FE_OK) return s;
+    if (density) *density = c->scene_iters[0][scene];
+    if (divergence) *divergence = c->scene_iters[1][scene];
+    if (density_total) *density_total = c->scene_total_iters[0][scene];
+    if (divergence_total) *divergence_total = c->scene_total_iters[1][scene];
+    return FE_OK;
+}
+
+fe_status fe_batch_use_graph(fe_ctx* c, int enable) {
+    FE_CHECK_CTX(c);
+    if (c->nscenes <= 1) return fail(FE_ERR_INVALID_ARGUMENT, "not a batch context (fe_batch_create)");
+    DeviceGuard g(c->device);
+    drop_graph(c);
+    c->batch_graph = enable != 0;
     return FE_OK;
 }
 

@@ -9,24 +9,71 @@ namespace fe {
 constexpr float kPi = 3.1415926535897932384626433832795f;
 constexpr uint32_t kP1 = 73856093u, kP2 = 19349663u, kP3 = 83492791u;
 
+// One scene's constants, as the kernels see them. A context holds one scene, or a batch of
+// scenes laid out one after another: scene s owns particles [part0, part0 + n) in every
+// particle order (position_a, hash order and schedule order keep scenes apart) and hash
+// table entries [tab0, tab0 + T).
 struct Sph {
     float h, hh, mass, k;   // k = 8 / (pi h^3) of common.glsl:42-43 / :61-62
     float inv_h;            // 1 / h: the cell coordinate is floor(pos * (1 / h))
     uint32_t n, mask;       // particles, T - 1 (T is a power of two, so % T == & (T - 1))
+    uint32_t tab0, part0;   // first hash-table entry and first particle of the scene
+    float dt, rho0, nu, omega, rad;
+    float gx, gy, gz;
+    float mnx, mny, mnz, mxx, mxy, mxz;   // box_min, box_max
+    int c0x, c0y, c0z;                     // origin cell of the Morton schedule key
+    unsigned long long thr[2];             // converge thresholds (density, divergence)
 };
 
-inline Sph sph_of(const fe_ctx* c) {
+// The scene table of a batch (tab == nullptr: one scene, whose constants are passed by value).
+struct Scenes {
+    const Sph* tab;
+    const uint8_t* sid;   // scene of each particle index (the same in every particle order)
+    uint32_t count;
+};
+
+__device__ __forceinline__ Sph scene_at(const Scenes& sc, const Sph& s0, uint32_t idx) {
+    return sc.tab ? sc.tab[sc.sid[idx]] : s0;
+}
+__device__ __forceinline__ uint32_t scene_id(const Scenes& sc, uint32_t idx) { return sc.tab ? sc.sid[idx] : 0u; }
+
+// Scene constants from a parameter block (host side).
+inline Sph sph_from(const fe_params& p, uint32_t n, uint32_t T, uint32_t tab0, uint32_t part0) {
     Sph s;
-    s.h = c->params.smoothing_radius;
+    s.h = p.smoothing_radius;
     s.hh = s.h * s.h;
-    s.mass = c->params.particle_mass;
+    s.mass = p.particle_mass;
     const float h3 = s.h * s.h * s.h;
     s.k = 8.0f / (kPi * h3);
     s.inv_h = 1.0f / s.h;
-    s.n = c->n;
-    s.mask = c->T - 1u;
+    s.n = n;
+    s.mask = T - 1u;
+    s.tab0 = tab0;
+    s.part0 = part0;
+    s.dt = p.dt;
+    s.rho0 = p.target_density;
+    s.nu = p.viscosity;
+    s.omega = p.relax_factor;
+    s.rad = p.particle_radius;
+    s.gx = p.gravity[0];
+    s.gy = p.gravity[1];
+    s.gz = p.gravity[2];
+    s.mnx = p.box_min[0];
+    s.mny = p.box_min[1];
+    s.mnz = p.box_min[2];
+    s.mxx = p.box_max[0];
+    s.mxy = p.box_max[1];
+    s.mxz = p.box_max[2];
+    s.c0x = (int)floorf(p.box_min[0] * s.inv_h) - 1;
+    s.c0y = (int)floorf(p.box_min[1] * s.inv_h) - 1;
+    s.c0z = (int)floorf(p.box_min[2] * s.inv_h) - 1;
+    s.thr[0] = s.thr[1] = 0ull;
     return s;
 }
+
+inline Sph sph_of(const fe_ctx* c) { return sph_from(c->params, c->n, c->T, 0u, 0u); }
+
+inline Scenes scenes_of(const fe_ctx* c) { return Scenes{c->scene_tab, c->sid, c->nscenes}; }
 
 __device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, uint32_t mask) {
     return ((uint32_t)cx * kP1 ^ (uint32_t)cy * kP2 ^ (uint32_t)cz * kP3) & mask;

@@ -54,7 +54,7 @@ __device__ __forceinline__ void visit27(const uint2* __restrict__ range, const f
 #pragma unroll
         for (int y = 0; y < 3; y++)
 #pragma unroll
-            for (int x = 0; x < 3; x++) rr[y * 3 + x] = range[(hx[x] ^ hy[y] ^ hz[z]) & s.mask];
+            for (int x = 0; x < 3; x++) rr[y * 3 + x] = range[s.tab0 + ((hx[x] ^ hy[y] ^ hz[z]) & s.mask)];
 #pragma unroll
         for (int b = 0; b < 9; b++) {
             if (rr[b].x == kSentinel) continue;
@@ -166,7 +166,7 @@ __device__ void build_group(const Sph& s, const uint2* __restrict__ range, const
     uint32_t bst = 0, bcnt = 0;
     if (lane < 27) {
         const int dz = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dx = lane % 3 - 1;
-        const uint2 r = range[cell_hash(gc.x + dx, gc.y + dy, gc.z + dz, s.mask)];
+        const uint2 r = range[s.tab0 + cell_hash(gc.x + dx, gc.y + dy, gc.z + dz, s.mask)];
         if (r.x != kSentinel) {
             bst = r.x;
             bcnt = r.y - r.x;
@@ -226,7 +226,7 @@ __device__ void build_group(const Sph& s, const uint2* __restrict__ range, const
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_build_list(Sph s, const uint2* __restrict__ range,
+__global__ __launch_bounds__(kBlock) void k_build_list(Sph s0, Scenes sc, uint32_t n, const uint2* __restrict__ range,
                                                        const float4* __restrict__ pos_b,
                                                        const float4* __restrict__ pos_m,
                                                        const uint32_t* __restrict__ inv, uint32_t* __restrict__ nbr,
@@ -234,35 +234,43 @@ __global__ __launch_bounds__(kBlock) void k_build_list(Sph s, const uint2* __res
     __shared__ BuildStage stage[kWaves];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t base = (xcd_block() * kWaves + w) * 64u;
-    if (base >= s.n) return;
+    if (base >= n) return;
     const uint32_t t = base + lane;
-    const bool valid = t < s.n;
+    const bool valid = t < n;
+    // a group is a run of work-items of one scene whose particles share a cell
+    const uint32_t sid = scene_id(sc, valid ? t : base);
+    const float inv_h = scene_at(sc, s0, valid ? t : base).inv_h;
     const float4 p = valid ? pos_m[t] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const int3 c = cell_of(p.x, p.y, p.z, s.inv_h);
+    const int3 c = cell_of(p.x, p.y, p.z, inv_h);
     int3 cp = make_int3(__shfl_up(c.x, 1, 64), __shfl_up(c.y, 1, 64), __shfl_up(c.z, 1, 64));
+    uint32_t sp = __shfl_up(sid, 1, 64);
     if (lane == 0 && base > 0) {
         const float4 q = pos_m[base - 1];
-        cp = cell_of(q.x, q.y, q.z, s.inv_h);
+        sp = scene_id(sc, base - 1);
+        cp = cell_of(q.x, q.y, q.z, scene_at(sc, s0, base - 1).inv_h);
     }
-    const bool start = valid && (t == 0 || c.x != cp.x || c.y != cp.y || c.z != cp.z);
+    const bool start = valid && (t == 0 || c.x != cp.x || c.y != cp.y || c.z != cp.z || sid != sp);
     uint64_t starts = __ballot(start);
     while (starts) {
         const int sl = __builtin_ctzll(starts);
         starts &= starts - 1;
         const uint32_t gs = base + (uint32_t)sl;
         const int3 gc = make_int3(__shfl(c.x, sl, 64), __shfl(c.y, sl, 64), __shfl(c.z, sl, 64));
+        const uint32_t gsid = __shfl(sid, sl, 64);
+        const Sph s = scene_at(sc, s0, gs);
         uint32_t ge;
         if (starts) {
             ge = base + (uint32_t)__builtin_ctzll(starts);
         } else {
             // the group runs to the end of the window, and possibly beyond it
-            ge = base + 64u < s.n ? base + 64u : s.n;
-            while (ge < s.n) {
+            ge = base + 64u < n ? base + 64u : n;
+            while (ge < n) {
                 const uint32_t j = ge + lane;
-                const bool vj = j < s.n;
+                const bool vj = j < n;
                 const float4 q = vj ? pos_m[j] : make_float4(0.f, 0.f, 0.f, 0.f);
                 const int3 cj = cell_of(q.x, q.y, q.z, s.inv_h);
-                const uint64_t diff = __ballot(!vj || cj.x != gc.x || cj.y != gc.y || cj.z != gc.z);
+                const uint64_t diff = __ballot(!vj || cj.x != gc.x || cj.y != gc.y || cj.z != gc.z ||
+                                               scene_id(sc, vj ? j : 0) != gsid);
                 if (diff) {
                     ge += (uint32_t)__builtin_ctzll(diff);
                     break;
@@ -306,7 +314,7 @@ struct DensityAlpha {
             a.sgs += mx * mx + my * my + mz * mz;
         }
     }
-    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&) const {
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         dens[i] = a.rho;
         dens_m[t] = a.rho;
         const float alpha = a.sgs + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
@@ -321,7 +329,6 @@ struct Viscosity {
     const float* dens_m;
     float4* vel_a;
     float4* va_m;
-    float nu, gdx, gdy, gdz, dt;
     struct Self { float4 v; };
     struct Cand { float4 v; float rho; };
     struct Acc { float x, y, z; };
@@ -340,9 +347,9 @@ struct Viscosity {
             a.z += (m_rho * (c.v.z - si.v.z)) * w;
         }
     }
-    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si) const {
-        const float vx = si.v.x + nu * a.x, vy = si.v.y + nu * a.y, vz = si.v.z + nu * a.z;
-        const float4 v = make_float4(vx + gdx * dt, vy + gdy * dt, vz + gdz * dt, 0.0f);
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si, const Sph& s) const {
+        const float vx = si.v.x + s.nu * a.x, vy = si.v.y + s.nu * a.y, vz = si.v.z + s.nu * a.z;
+        const float4 v = make_float4(vx + s.gx * s.dt, vy + s.gy * s.dt, vz + s.gz * s.dt, 0.0f);
         vel_a[i] = v;
         va_m[t] = v;
         return 0;
@@ -356,7 +363,6 @@ struct DensitySource {
     float* src;
     float* pres;
     float* prho_m;
-    float dt, rho0;
     struct Self { float4 v; };
     struct Cand { float4 v; };
     struct Acc { float div; };
@@ -373,10 +379,10 @@ struct DensitySource {
             a.div += s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz));
         }
     }
-    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&) const {
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         const float rho = dens[i];
         float source = 0.0f;
-        if (dt > 1e-6f) source = ((rho0 - rho) / dt) - a.div;
+        if (s.dt > 1e-6f) source = ((s.rho0 - rho) / s.dt) - a.div;
         src[i] = source;
         pres[i] = 0.0f;
         prho_m[t] = prho_of(0.0f, rho);
@@ -411,7 +417,7 @@ struct DivergenceSource {
             a.div += s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz));
         }
     }
-    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&) const {
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         src[i] = -a.div;
         prho_m[t] = prho_of(pres[i], dens[i]);
         return 0;
@@ -491,18 +497,17 @@ struct DensitySourceFeat {
         }
         feat_pair(a.f, s, rx, ry, rz, r2, r, dx, dy, dz, c.pp);
     }
-    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si) const {
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si, const Sph& s) const {
         const float rho = base.dens[i];
         float source = 0.0f;
-        if (base.dt > 1e-6f) source = ((base.rho0 - rho) / base.dt) - a.div;
+        if (s.dt > 1e-6f) source = ((s.rho0 - rho) / s.dt) - a.div;
         base.src[i] = source;
         base.pres[i] = 0.0f;
         base.prho_m[t] = prho_of(0.0f, rho);
-        feat_write(feat, n_, t, sph_, a.f, rho, fac[i], source, si.pp, base.rho0, base.dt);
+        feat_write(feat, n_, t, s, a.f, rho, fac[i], source, si.pp, s.rho0, s.dt);
         return 0;
     }
     uint32_t n_;
-    Sph sph_;
 };
 
 // DivergenceSource plus the features of the divergence solve (velocity_a read at the
@@ -512,7 +517,6 @@ struct DivergenceSourceFeat {
     const float* fac;
     const float* pprev_m;
     float* feat;
-    float rho0, dt;
     struct Self { float4 v; float pp; };
     struct Cand { float4 v; float pp; };
     struct Acc { float div; FeatAcc f; };
@@ -530,16 +534,15 @@ struct DivergenceSourceFeat {
         }
         feat_pair(a.f, s, rx, ry, rz, r2, r, dx, dy, dz, c.pp);
     }
-    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si) const {
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si, const Sph& s) const {
         const float source = -a.div;
         base.src[i] = source;
         const float rho = base.dens[i];
         base.prho_m[t] = prho_of(base.pres[i], rho);
-        feat_write(feat, n_, t, sph_, a.f, rho, fac[i], source, si.pp, rho0, dt);
+        feat_write(feat, n_, t, s, a.f, rho, fac[i], source, si.pp, s.rho0, s.dt);
         return 0;
     }
     uint32_t n_;
-    Sph sph_;
 };
 
 // pressure_force.comp:15-78. Neighbours supply p_j / rho_j^2 (kNoRho when rho_j <= 1e-6,
@@ -573,7 +576,7 @@ struct PressureForce {
             a.z += mt * (g * rz);
         }
     }
-    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&) const {
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         const float4 v = make_float4(-a.x, -a.y, -a.z, 0.0f);
         pacc[i] = v;
         pacc_m[t] = v;
@@ -590,7 +593,6 @@ struct PressureUpdate {
     const float* dens;
     float* pres;
     float* prho_m;
-    float dt, omega, rho0;
     struct Self { float4 a; };
     struct Cand { float4 a; };
     struct Acc { float ap; };
@@ -607,16 +609,16 @@ struct PressureUpdate {
             a.ap += s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz));
         }
     }
-    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&) const {
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         const float alpha_i = fac[i], rho_i = dens[i], src_i = src[i];
         float p = pres[i];
-        const float ap = dt * a.ap;
+        const float ap = s.dt * a.ap;
         const float err = src_i - ap;
-        const unsigned long long q = residual_q(err, dt, rho0);
+        const unsigned long long q = residual_q(err, s.dt, s.rho0);
         if (alpha_i > 1e-6f && rho_i > 1e-6f) {
-            const float a_ii = -(dt / (rho_i * rho_i)) * alpha_i;
+            const float a_ii = -(s.dt / (rho_i * rho_i)) * alpha_i;
             if (fabsf(a_ii) > 1e-20f) {
-                const float corr = (omega * err) / a_ii;
+                const float corr = (s.omega * err) / a_ii;
                 const float np = p + corr;
                 p = np > 0.0f ? np : 0.0f;
                 pres[i] = p;
@@ -645,7 +647,8 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v
 // converged previous sweep skips the launch; with `partial` set, each block stores the sum
 // of its particles' finish() values.
 template <class P>
-__global__ __launch_bounds__(kBlock) void k_list_gather(P pol, Sph s, const uint2* __restrict__ range,
+__global__ __launch_bounds__(kBlock) void k_list_gather(P pol, Sph s0, Scenes sc, uint32_t n,
+                                                        const uint2* __restrict__ range,
                                                         const float4* __restrict__ pos_b,
                                                         const float4* __restrict__ pos_m,
                                                         const uint32_t* __restrict__ sched,
@@ -654,11 +657,22 @@ __global__ __launch_bounds__(kBlock) void k_list_gather(P pol, Sph s, const uint
                                                         const uint32_t* __restrict__ cnt, uint32_t list_ok,
                                                         unsigned long long* __restrict__ partial,
                                                         const unsigned long long* __restrict__ prev,
-                                                        unsigned long long thr) {
-    if (prev && *prev <= thr) return;
+                                                        unsigned long long thr, uint32_t kind,
+                                                        unsigned long long* __restrict__ slot_out) {
+    // one scene: the sweep returns at once when the previous sweep converged; a batch: each
+    // particle skips when its own scene converged (slots are kMaxSolveIters apart per scene)
+    if (!sc.tab && prev && *prev <= thr) return;
     const uint32_t t = work_item();
     unsigned long long q = 0;
-    if (t < s.n) {
+    uint32_t scene = 0;
+    bool act = t < n;
+    Sph s = s0;
+    if (act && sc.tab) {
+        scene = sc.sid[t];
+        s = sc.tab[scene];
+        if (prev && prev[(size_t)scene * kMaxSolveIters] <= s.thr[kind]) act = false;
+    }
+    if (act) {
         const uint32_t i = sched[t];
         const float4 pi = pos_m[t];
         const typename P::Self si = pol.self(i, t);
@@ -686,11 +700,24 @@ __global__ __launch_bounds__(kBlock) void k_list_gather(P pol, Sph s, const uint
                 pol.pair(acc, si, s, pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, pol.load(inv[j]));
             });
         }
-        q = pol.finish(i, t, acc, si);
+        q = pol.finish(i, t, acc, si, s);
     }
     if (partial) {
         const unsigned long long b = block_sum_u64(q);
         if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    }
+    if (slot_out) {
+        // per-scene residual sums by atomics (integer, so order-independent): one per wave
+        // unless the wave straddles two scenes
+        const uint32_t s_first = __shfl(scene, 0, 64);
+        if (__all(scene == s_first)) {
+            unsigned long long v = q;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if ((threadIdx.x & 63) == 0 && v) atomicAdd(slot_out + (size_t)s_first * kMaxSolveIters, v);
+        } else if (q) {
+            atomicAdd(slot_out + (size_t)scene * kMaxSolveIters, q);
+        }
     }
 }
 
@@ -730,11 +757,12 @@ hipError_t need(fe_ctx* c, uint32_t bits) {
 
 template <class P>
 hipError_t run_list(fe_ctx* c, const char* name, const P& pol, unsigned long long* partial = nullptr,
-                    const unsigned long long* prev = nullptr, unsigned long long thr = 0) {
+                    const unsigned long long* prev = nullptr, unsigned long long thr = 0, uint32_t kind = 0,
+                    unsigned long long* slot_out = nullptr) {
     Launch L(c, name);
-    hipLaunchKernelGGL(k_list_gather<P>, grid_for(c->n), dim3(kBlock), 0, c->stream, pol, sph_of(c), c->range,
-                       c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr, c->nbr_count, c->list_valid ? 1u : 0u,
-                       partial, prev, thr);
+    hipLaunchKernelGGL(k_list_gather<P>, grid_for(c->n), dim3(kBlock), 0, c->stream, pol, sph_of(c), scenes_of(c),
+                       c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr, c->nbr_count,
+                       c->list_valid ? 1u : 0u, partial, prev, thr, kind, slot_out);
     return hipGetLastError();
 }
 
@@ -764,8 +792,8 @@ hipError_t launch_density_alpha(fe_ctx* c) {
     if (e != hipSuccess) return e;
     {
         Launch L(c, "build_list");
-        hipLaunchKernelGGL(k_build_list, grid_for(c->n), dim3(kBlock), 0, c->stream, sph_of(c), c->range, c->pos_b,
-                           c->pos_m, c->inv, c->nbr, c->nbr_count);
+        hipLaunchKernelGGL(k_build_list, grid_for(c->n), dim3(kBlock), 0, c->stream, sph_of(c), scenes_of(c), c->n,
+                           c->range, c->pos_b, c->pos_m, c->inv, c->nbr, c->nbr_count);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -780,8 +808,7 @@ hipError_t launch_density_alpha(fe_ctx* c) {
 hipError_t launch_viscosity(fe_ctx* c) {
     hipError_t e = need(c, kMirrorPos | kMirrorVelB | kMirrorDens);
     if (e != hipSuccess) return e;
-    const fe_params& p = c->params;
-    Viscosity pol{c->vb_m, c->dens_m, c->vel_a, c->va_m, p.viscosity, p.gravity[0], p.gravity[1], p.gravity[2], p.dt};
+    Viscosity pol{c->vb_m, c->dens_m, c->vel_a, c->va_m};
     e = run_list(c, "viscosity", pol);
     c->mirrors |= kMirrorVelA;
     return e;
@@ -790,10 +817,10 @@ hipError_t launch_viscosity(fe_ctx* c) {
 hipError_t launch_density_source(fe_ctx* c) {
     hipError_t e = need(c, kMirrorPos | kMirrorVelA);
     if (e != hipSuccess) return e;
-    DensitySource pol{c->va_m, c->dens, c->src, c->pres, c->prho_m, c->params.dt, c->params.target_density};
+    DensitySource pol{c->va_m, c->dens, c->src, c->pres, c->prho_m};
     if (learn_features_wanted(c, 0)) {
         if ((e = learn_prev(c, 0)) != hipSuccess) return e;
-        DensitySourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n, sph_of(c)};
+        DensitySourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n};
         e = run_list(c, "density_source_feat", fp);
         learn_features_done(c, 0);
     } else {
@@ -809,8 +836,7 @@ hipError_t launch_divergence_source(fe_ctx* c) {
     DivergenceSource pol{c->vel_a, c->sched[0], c->dens, c->pres, c->src, c->prho_m};
     if (learn_features_wanted(c, 1)) {
         if ((e = learn_prev(c, 1)) != hipSuccess) return e;
-        DivergenceSourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->params.target_density, c->params.dt,
-                                c->n, sph_of(c)};
+        DivergenceSourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n};
         e = run_list(c, "divergence_source_feat", fp);
         learn_features_done(c, 1);
     } else {
@@ -820,22 +846,23 @@ hipError_t launch_divergence_source(fe_ctx* c) {
     return e;
 }
 
-hipError_t launch_pressure_force(fe_ctx* c, const unsigned long long* prev, unsigned long long thr) {
+hipError_t launch_pressure_force(fe_ctx* c, const unsigned long long* prev, unsigned long long thr, uint32_t kind) {
     hipError_t e = need(c, kMirrorPos | kMirrorPrho);
     if (e != hipSuccess) return e;
     PressureForce pol{c->pres, c->dens, c->prho_m, c->pacc, c->pacc_m};
-    e = run_list(c, "pressure_force", pol, nullptr, prev, thr);
+    e = run_list(c, "pressure_force", pol, nullptr, prev, thr, kind);
     c->mirrors |= kMirrorPacc;
     return e;
 }
 
 hipError_t launch_pressure_update(fe_ctx* c, unsigned long long* out, const unsigned long long* prev,
-                                  unsigned long long thr) {
+                                  unsigned long long thr, uint32_t kind) {
     hipError_t e = need(c, kMirrorPos | kMirrorPacc | kMirrorPrho);
     if (e != hipSuccess) return e;
-    PressureUpdate pol{c->pacc_m, c->fac, c->src, c->dens, c->pres, c->prho_m, c->params.dt, c->params.relax_factor,
-                       c->params.target_density};
-    e = run_list(c, "pressure_update", pol, out ? c->partial : nullptr, prev, thr);
+    PressureUpdate pol{c->pacc_m, c->fac, c->src, c->dens, c->pres, c->prho_m};
+    if (c->nscenes > 1)   // per-scene sums straight into the scenes' slots
+        return run_list(c, "pressure_update", pol, nullptr, prev, thr, kind, out);
+    e = run_list(c, "pressure_update", pol, out ? c->partial : nullptr, prev, thr, kind);
     if (e != hipSuccess || !out) return e;
     Launch L(c, "resid_finalize");
     hipLaunchKernelGGL(k_resid_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->partial, gather_partials(c->n), out,

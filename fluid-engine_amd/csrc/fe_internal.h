@@ -92,6 +92,7 @@ struct LearnState {
 };
 
 struct Launch;
+struct Sph;
 
 }  // namespace fe
 
@@ -102,9 +103,23 @@ struct fe_ctx {
     hipStream_t stream = nullptr;
     fe_params params{};
     fe_options options{};
-    uint32_t n = 0;      // particles
-    uint32_t T = 0;      // hash-table entries = next_power_of_two(n)
-    uint32_t log2T = 0;
+    uint32_t n = 0;      // particles (all scenes)
+    uint32_t T = 0;      // hash-table entries = next_power_of_two(n) (the sum over scenes)
+    uint32_t log2T = 0;  // bits of a table index: log2(T) rounded up
+
+    // A batch of independent scenes (nscenes > 1): per-scene constants on the device, the
+    // scene of each particle index, and the host copies of the layout.
+    uint32_t nscenes = 1;
+    fe::Sph* scene_tab = nullptr;
+    uint8_t* sid = nullptr;
+    std::vector<fe_params> scene_params;
+    std::vector<uint32_t> scene_n, scene_T, scene_part0, scene_tab0;
+    std::vector<uint32_t> scene_iters[2];       // sweeps of the last solve, per scene
+    std::vector<uint64_t> scene_total_iters[2];
+    uint32_t* scene_stats = nullptr;            // 3 per scene
+    bool batch_graph = false;                   // replay the frame from a captured graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
 
     // GpuPhysicsData fields
     float4* pos_a = nullptr;
@@ -135,6 +150,7 @@ struct fe_ctx {
     uint32_t* sched[2] = {nullptr, nullptr};
     uint32_t* mkeys[2] = {nullptr, nullptr};
     uint32_t morton_bits = 30;   // bits of the Morton key (3 x bits per axis)
+    uint32_t scene_bits = 0;     // bits of the scene number above them (a batch)
     uint32_t* inv = nullptr;     // inverse schedule: hash index -> schedule position
     float4* pos_m = nullptr;     // position_b by schedule position
     float4* vb_m = nullptr;      // velocity_b
@@ -224,9 +240,12 @@ hipError_t launch_rebuild_ends(fe_ctx* c);
 hipError_t launch_density_alpha(fe_ctx* c);
 hipError_t launch_viscosity(fe_ctx* c);
 hipError_t launch_density_source(fe_ctx* c);
-hipError_t launch_pressure_force(fe_ctx* c, const unsigned long long* prev, unsigned long long thr);
+// Jacobi sweep kernels. prev: the previous sweep's residual slot (nullptr: no early exit);
+// thr: the threshold of one scene; kind selects a batch's per-scene threshold (0 density,
+// 1 divergence). A batch's slots are kMaxSolveIters apart per scene.
+hipError_t launch_pressure_force(fe_ctx* c, const unsigned long long* prev, unsigned long long thr, uint32_t kind);
 hipError_t launch_pressure_update(fe_ctx* c, unsigned long long* out, const unsigned long long* prev,
-                                  unsigned long long thr);
+                                  unsigned long long thr, uint32_t kind);
 hipError_t launch_pressure_integration(fe_ctx* c);
 hipError_t launch_divergence_source(fe_ctx* c);
 hipError_t launch_divergence_integration(fe_ctx* c);

@@ -32,6 +32,7 @@ fe_status check_net(int net) {
 fe_status ensure_alloc(fe_ctx* c) {
     fe::LearnState& w = c->ws;
     if (w.alloc) return FE_OK;
+    if (c->nscenes > 1) return fail(FE_ERR_UNSUPPORTED, "the learned warm start takes one scene per context");
     const size_t n = c->n;
     FE_HIP(hipMalloc(&w.feat, sizeof(float) * kMlpIn * n), "hipMalloc features");
     FE_HIP(hipMalloc(&w.pprev, sizeof(float) * n), "hipMalloc pprev");

@@ -267,6 +267,17 @@ void sort_scratch_free(SortScratch* s) {
     s->counts = s->totals = nullptr;
 }
 
+namespace {
+__global__ __launch_bounds__(256) void k_copy_pairs(const uint32_t* __restrict__ ks, const uint32_t* __restrict__ vs,
+                                                    uint32_t* __restrict__ kd, uint32_t* __restrict__ vd, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        kd[i] = ks[i];
+        vd[i] = vs[i];
+    }
+}
+}  // namespace
+
 hipError_t radix_sort_pairs(fe_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
                             const char* tag) {
     if (n <= 1) return hipSuccess;
@@ -291,6 +302,15 @@ hipError_t radix_sort_pairs(fe_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uin
             hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, c->stream, keys[0], vals[0],
                                keys[1], vals[1], n, shift, tiles, c->sort.counts, c->sort.totals);
         }
+        std::swap(keys[0], keys[1]);
+        std::swap(vals[0], vals[1]);
+    }
+    if ((passes & 1u) && c->batch_graph) {
+        // A frame replayed from a graph uses the buffers of its capture: bring an odd pass
+        // count's result back to the buffer the sort started from.
+        Launch L(c, "radix_home");
+        hipLaunchKernelGGL(k_copy_pairs, dim3((n + 255) / 256), dim3(256), 0, c->stream, keys[0], vals[0], keys[1],
+                           vals[1], n);
         std::swap(keys[0], keys[1]);
         std::swap(vals[0], vals[1]);
     }

@@ -20,16 +20,20 @@ __device__ __forceinline__ uint32_t gid() { return blockIdx.x * kBlock + threadI
 // table is not read between the hash and the offsets). The cell coordinate
 // ivec3(floor(pos / h)) is evaluated as floor(pos * (1 / h)), the form the reference GPU
 // executed (it reproduces the bucket order of the reference's rest-density capture).
-__global__ __launch_bounds__(kBlock) void k_hash(const float4* __restrict__ pos, uint32_t n, uint32_t T,
-                                                 float inv_h, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+// In a batch the key is the scene's first table entry plus the scene's hash, so one stable
+// sort of all keys orders every scene exactly as its own sort would, within its own range.
+__global__ __launch_bounds__(kBlock) void k_hash(const float4* __restrict__ pos, uint32_t n, uint32_t T, Sph s0,
+                                                 Scenes sc, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                  uint32_t* __restrict__ keys_alt, uint32_t* __restrict__ vals_alt,
                                                  uint2* __restrict__ range) {
     uint32_t i = gid();
     if (i >= T) return;
     range[i] = make_uint2(kSentinel, 0u);
     if (i < n) {
+        const Sph s = scene_at(sc, s0, i);
         float4 p = pos[i];
-        keys[i] = cell_hash((int)floorf(p.x * inv_h), (int)floorf(p.y * inv_h), (int)floorf(p.z * inv_h), T - 1u);
+        keys[i] = s.tab0 + cell_hash((int)floorf(p.x * s.inv_h), (int)floorf(p.y * s.inv_h),
+                                     (int)floorf(p.z * s.inv_h), s.mask);
         vals[i] = i;
     } else {
         keys[i] = kSentinel;
@@ -71,16 +75,19 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
     return v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_morton_keys(const float4* __restrict__ pos, uint32_t n, float inv_h,
-                                                        int3 c0, uint32_t mask, uint32_t* __restrict__ keys,
+// In a batch the scene number sits above the 3 x bits Morton bits, which keeps scenes apart.
+__global__ __launch_bounds__(kBlock) void k_morton_keys(const float4* __restrict__ pos, uint32_t n, Sph s0, Scenes sc,
+                                                        uint32_t bits, uint32_t* __restrict__ keys,
                                                         uint32_t* __restrict__ idx) {
     const uint32_t i = gid();
     if (i >= n) return;
+    const Sph s = scene_at(sc, s0, i);
+    const uint32_t mask = (1u << bits) - 1u;
     const float4 p = pos[i];
-    const int3 c = cell_of(p.x, p.y, p.z, inv_h);
-    const uint32_t x = (uint32_t)(c.x - c0.x) & mask, y = (uint32_t)(c.y - c0.y) & mask,
-                   z = (uint32_t)(c.z - c0.z) & mask;
-    keys[i] = spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+    const int3 c = cell_of(p.x, p.y, p.z, s.inv_h);
+    const uint32_t x = (uint32_t)(c.x - s.c0x) & mask, y = (uint32_t)(c.y - s.c0y) & mask,
+                   z = (uint32_t)(c.z - s.c0z) & mask;
+    keys[i] = (scene_id(sc, i) << (3u * bits)) | spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
     idx[i] = i;
 }
 
@@ -121,10 +128,13 @@ __global__ __launch_bounds__(kBlock) void k_rebuild_ends(const uint32_t* __restr
 
 // pressure_integration.comp:16-77 (position_b -> position_a; collision order y, x, z).
 __global__ __launch_bounds__(kBlock) void k_pressure_integration(
-    uint32_t n, const float4* __restrict__ pos_b, const float4* __restrict__ pacc, float4* __restrict__ vel,
-    float4* __restrict__ pos_a, float dt, float rad, float3 mn, float3 mx) {
+    uint32_t n, Sph s0, Scenes sc, const float4* __restrict__ pos_b, const float4* __restrict__ pacc,
+    float4* __restrict__ vel, float4* __restrict__ pos_a) {
     uint32_t i = gid();
     if (i >= n) return;
+    const Sph s = scene_at(sc, s0, i);
+    const float dt = s.dt, rad = s.rad;
+    const float3 mn = make_float3(s.mnx, s.mny, s.mnz), mx = make_float3(s.mxx, s.mxy, s.mxz);
     const float4 p = pos_b[i], v = vel[i], a = pacc[i];
     float vx = v.x + a.x * dt, vy = v.y + a.y * dt, vz = v.z + a.z * dt;
     float px = p.x + vx * dt, py = p.y + vy * dt, pz = p.z + vz * dt;
@@ -151,11 +161,13 @@ __global__ __launch_bounds__(kBlock) void k_pressure_integration(
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
 
 // divergence_integration.comp:14-43.
-__global__ __launch_bounds__(kBlock) void k_divergence_integration(uint32_t n, float4* __restrict__ vel,
+__global__ __launch_bounds__(kBlock) void k_divergence_integration(uint32_t n, Sph s0, Scenes sc,
+                                                                   float4* __restrict__ vel,
                                                                    const float4* __restrict__ pacc,
-                                                                   float4* __restrict__ colors, float dt) {
+                                                                   float4* __restrict__ colors) {
     uint32_t i = gid();
     if (i >= n) return;
+    const float dt = scene_at(sc, s0, i).dt;
     const float4 v = vel[i], a = pacc[i];
     float vx = v.x + a.x * dt, vy = v.y + a.y * dt, vz = v.z + a.z * dt;
     float speed = sqrtf(vx * vx + vy * vy + vz * vz);
@@ -254,6 +266,49 @@ __global__ __launch_bounds__(kBlock) void k_stats_final(const uint32_t* __restri
     }
 }
 
+// stats.comp:18-30 for a batch: each scene's three values by atomics (max and wrapping u32
+// sums are order-independent, as in the reference), one set per wave unless the wave
+// straddles two scenes. The per-scene triples are zeroed first (k_fill_u32).
+__global__ __launch_bounds__(kBlock) void k_stats_scenes(uint32_t n, Scenes sc, const float4* __restrict__ vel,
+                                                         const float* __restrict__ dens,
+                                                         const float* __restrict__ src,
+                                                         uint32_t* __restrict__ stats) {
+    const uint32_t i = gid();
+    uint32_t mx = 0, s1 = 0, s2 = 0, scene = 0;
+    if (i < n) {
+        scene = sc.sid[i];
+        const float rho0 = sc.tab[scene].rho0;
+        const float4 v = vel[i];
+        const float speed = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+        mx = __float_as_uint(speed);
+        s1 = min(to_u32_sat(fabsf(dens[i] - rho0) * 1.0f), 10000u);
+        s2 = min(to_u32_sat(fabsf(src[i]) * 10.0f), 10000u);
+    }
+    const uint32_t first = __shfl(scene, 0, 64);
+    if (__all(scene == first)) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+            s1 += (uint32_t)__shfl_xor((int)s1, off, 64);
+            s2 += (uint32_t)__shfl_xor((int)s2, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMax(stats + 3 * first, mx);
+            atomicAdd(stats + 3 * first + 1, s1);
+            atomicAdd(stats + 3 * first + 2, s2);
+        }
+    } else if (i < n) {
+        atomicMax(stats + 3 * scene, mx);
+        atomicAdd(stats + 3 * scene + 1, s1);
+        atomicAdd(stats + 3 * scene + 2, s2);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, uint32_t count) {
+    uint32_t i = gid();
+    if (i < count) p[i] = 0u;
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_u64(unsigned long long* p, uint32_t count) {
     uint32_t i = gid();
     if (i < count) p[i] = 0ull;
@@ -295,8 +350,8 @@ inline dim3 grid_for(uint32_t count) { return dim3((count + kBlock - 1) / kBlock
 
 hipError_t launch_hash(fe_ctx* c) {
     Launch L(c, "spatial_hash");
-    hipLaunchKernelGGL(k_hash, grid_for(c->T), dim3(kBlock), 0, c->stream, c->pos_a, c->n, c->T,
-                       1.0f / c->params.smoothing_radius, c->keys[0], c->vals[0], c->keys[1], c->vals[1], c->range);
+    hipLaunchKernelGGL(k_hash, grid_for(c->T), dim3(kBlock), 0, c->stream, c->pos_a, c->n, c->T, sph_of(c),
+                       scenes_of(c), c->keys[0], c->vals[0], c->keys[1], c->vals[1], c->range);
     return hipGetLastError();
 }
 
@@ -308,14 +363,9 @@ hipError_t launch_offsets_reorder(fe_ctx* c) {
 }
 
 hipError_t launch_morton_keys(fe_ctx* c) {
-    const fe_params& p = c->params;
-    const float ih = 1.0f / p.smoothing_radius;
-    const int3 c0 = make_int3((int)floorf(p.box_min[0] * ih) - 1, (int)floorf(p.box_min[1] * ih) - 1,
-                              (int)floorf(p.box_min[2] * ih) - 1);
-    const uint32_t bits = c->morton_bits / 3;
     Launch L(c, "morton_keys");
-    hipLaunchKernelGGL(k_morton_keys, grid_for(c->n), dim3(kBlock), 0, c->stream, c->pos_b, c->n, ih, c0,
-                       (1u << bits) - 1u, c->mkeys[0], c->sched[0]);
+    hipLaunchKernelGGL(k_morton_keys, grid_for(c->n), dim3(kBlock), 0, c->stream, c->pos_b, c->n, sph_of(c),
+                       scenes_of(c), c->morton_bits / 3, c->mkeys[0], c->sched[0]);
     return hipGetLastError();
 }
 
@@ -352,11 +402,8 @@ hipError_t launch_pressure_integration(fe_ctx* c) {
     c->ns_fresh = false;   // position_a moves
     c->da_fresh = false;
     Launch L(c, "pressure_integration");
-    const fe_params& p = c->params;
-    hipLaunchKernelGGL(k_pressure_integration, grid_for(c->n), dim3(kBlock), 0, c->stream, c->n, c->pos_b, c->pacc,
-                       c->vel_a, c->pos_a, p.dt, p.particle_radius,
-                       make_float3(p.box_min[0], p.box_min[1], p.box_min[2]),
-                       make_float3(p.box_max[0], p.box_max[1], p.box_max[2]));
+    hipLaunchKernelGGL(k_pressure_integration, grid_for(c->n), dim3(kBlock), 0, c->stream, c->n, sph_of(c),
+                       scenes_of(c), c->pos_b, c->pacc, c->vel_a, c->pos_a);
     return hipGetLastError();
 }
 
@@ -364,12 +411,20 @@ hipError_t launch_pressure_integration(fe_ctx* c) {
 hipError_t launch_divergence_integration(fe_ctx* c) {
     c->mirrors &= ~kMirrorVelA;
     Launch L(c, "divergence_integration");
-    hipLaunchKernelGGL(k_divergence_integration, grid_for(c->n), dim3(kBlock), 0, c->stream, c->n, c->vel_a, c->pacc,
-                       c->colors, c->params.dt);
+    hipLaunchKernelGGL(k_divergence_integration, grid_for(c->n), dim3(kBlock), 0, c->stream, c->n, sph_of(c),
+                       scenes_of(c), c->vel_a, c->pacc, c->colors);
     return hipGetLastError();
 }
 
 hipError_t launch_stats(fe_ctx* c) {
+    if (c->nscenes > 1) {
+        hipLaunchKernelGGL(k_fill_u32, grid_for(3 * c->nscenes), dim3(kBlock), 0, c->stream, c->scene_stats,
+                           3 * c->nscenes);
+        Launch L(c, "stats");
+        hipLaunchKernelGGL(k_stats_scenes, grid_for(c->n), dim3(kBlock), 0, c->stream, c->n, scenes_of(c), c->vel_a,
+                           c->dens, c->src, c->scene_stats);
+        return hipGetLastError();
+    }
     uint32_t blocks = (c->n + kBlock - 1) / kBlock;
     if (blocks > kStatsBlocks) blocks = kStatsBlocks;
     {

@@ -24,6 +24,7 @@ __all__ = [
     "default_options", "FE_SORT_BITONIC", "FE_SORT_RADIX", "FE_SOLVER_FIXED", "FE_SOLVER_CONVERGE",
     "FE_WARM_START_REFERENCE", "FE_WARM_START_LEARNED", "FIELDS", "EXPORTED_SYMBOLS", "LIB_PATH",
     "FE_NET_DENSITY", "FE_NET_DIVERGENCE", "FE_MLP_INPUTS", "FE_MLP_HIDDEN", "FE_MLP_PARAMS", "FE_MLP_NORM",
+    "Batch",
 ]
 
 
@@ -293,3 +294,70 @@ class Simulation:
 
     def learn_predict(self, net: int):
         check(self._lib.fe_learn_predict(self._ctx, net), "fe_learn_predict")
+
+
+class Batch(Simulation):
+    """Many independent scenes in one context (fe_batch_*): every kernel of a frame is one
+    launch over all scenes; each scene evolves as it would alone. Whole-context methods
+    (step, run, set_options, timers, profiling) are those of Simulation; per-scene fields,
+    stats and sweep counts take a scene index."""
+
+    def __init__(self, workloads, device: int = -1, options: FeOptions | None = None):
+        self._lib = load_library()
+        count = len(workloads)
+        params = (FeParams * count)(*[w.params for w in workloads])
+        pos = [np.ascontiguousarray(w.positions, dtype=np.float32).reshape(-1, 3) for w in workloads]
+        ptrs = (c_void_p * count)(*[p.ctypes.data for p in pos])
+        counts = (c_uint32 * count)(*[p.shape[0] for p in pos])
+        ctx = c_void_p()
+        check(self._lib.fe_batch_create(params, ptrs, counts, count, device, byref(ctx)), "fe_batch_create")
+        self._ctx = ctx
+        n = c_uint32()
+        t = c_uint32()
+        check(self._lib.fe_num_particles(ctx, byref(n), byref(t)), "fe_num_particles")
+        self.n = n.value
+        self.table_size = t.value
+        self.scenes = count
+        self.scene_n = []
+        self.scene_table_size = []
+        for k in range(count):
+            check(self._lib.fe_batch_scene_info(ctx, k, byref(n), byref(t)), "fe_batch_scene_info")
+            self.scene_n.append(n.value)
+            self.scene_table_size.append(t.value)
+        if options is not None:
+            self.set_options(options)
+
+    def _scene_shape(self, scene, name):
+        fid, per, table = FIELDS[name]
+        rows = self.scene_table_size[scene] if table else self.scene_n[scene]
+        return fid, (rows, per) if per > 1 else (rows,)
+
+    def download_scene(self, scene: int, name: str) -> np.ndarray:
+        fid, shape = self._scene_shape(scene, name)
+        out = np.empty(shape, dtype=np.uint32 if name in INT_FIELDS else np.float32)
+        check(self._lib.fe_batch_download(self._ctx, scene, fid, out.ctypes.data, out.nbytes), "fe_batch_download")
+        return out
+
+    def upload_scene(self, scene: int, name: str, data) -> None:
+        fid, shape = self._scene_shape(scene, name)
+        arr = np.ascontiguousarray(data, dtype=np.float32).reshape(shape)
+        check(self._lib.fe_batch_upload(self._ctx, scene, fid, arr.ctypes.data, arr.nbytes), "fe_batch_upload")
+
+    def set_scene_params(self, scene: int, params: FeParams):
+        check(self._lib.fe_batch_set_params(self._ctx, scene, byref(params)), "fe_batch_set_params")
+
+    def read_scene_stats(self, scene: int) -> np.ndarray:
+        out = (c_uint32 * 3)()
+        check(self._lib.fe_batch_read_stats(self._ctx, scene, out), "fe_batch_read_stats")
+        return np.array(out[:], dtype=np.uint32)
+
+    def scene_iterations(self, scene: int):
+        """(density, divergence) sweeps of the last frame and (density, divergence) totals."""
+        d, v = c_uint32(), c_uint32()
+        dt, vt = c_uint64(), c_uint64()
+        check(self._lib.fe_batch_iterations(self._ctx, scene, byref(d), byref(v), byref(dt), byref(vt)),
+              "fe_batch_iterations")
+        return (d.value, v.value), (dt.value, vt.value)
+
+    def use_graph(self, enable: bool = True):
+        check(self._lib.fe_batch_use_graph(self._ctx, 1 if enable else 0), "fe_batch_use_graph")

@@ -147,6 +147,16 @@ _SIGNATURES = [
     ("fe_learn_gradient", [c_void_p, c_int, c_void_p, c_void_p, c_uint32, c_void_p, POINTER(c_float)]),
     ("fe_learn_features", [c_void_p, c_int, c_void_p, c_void_p]),
     ("fe_learn_predict", [c_void_p, c_int]),
+    ("fe_batch_create", [POINTER(FeParams), c_void_p, c_void_p, c_uint32, c_int, POINTER(c_void_p)]),
+    ("fe_batch_scenes", [c_void_p, POINTER(c_uint32)]),
+    ("fe_batch_scene_info", [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32)]),
+    ("fe_batch_set_params", [c_void_p, c_uint32, POINTER(FeParams)]),
+    ("fe_batch_download", [c_void_p, c_uint32, c_int, c_void_p, c_size_t]),
+    ("fe_batch_upload", [c_void_p, c_uint32, c_int, c_void_p, c_size_t]),
+    ("fe_batch_read_stats", [c_void_p, c_uint32, POINTER(c_uint32)]),
+    ("fe_batch_iterations", [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint64),
+                             POINTER(c_uint64)]),
+    ("fe_batch_use_graph", [c_void_p, c_int]),
 ]
 
 EXPORTED_SYMBOLS = tuple(name for name, _ in _SIGNATURES)

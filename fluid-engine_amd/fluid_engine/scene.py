@@ -133,3 +133,24 @@ def rest_density_scene() -> Workload:
     params = make_params(float(r), float(mass), float(h), 1000.0, 0.15, 0.5, 0.005, 4, 4, (0.0, -9.81, 0.0),
                          (-1.5, 0.0, -1.0), (0.8, 4.0, 1.0))
     return Workload("rest_density_4800", pos, params)
+
+
+def batch_scenes(count=64, seed=0, n_min=4096, n_max=262144, **kw) -> list:
+    """Independent dam breaks of uneven sizes: scene k has about N_k = exp(U[ln n_min, ln n_max])
+    particles (a block of round(N_k^(1/3))^2 x round(N_k / that^2)), a cubic box of edge
+    (N_k / 32768)^(1/3) m, rest density U[950, 1050] and viscosity U[0.005, 0.05] (mass =
+    rho0 s^3), all drawn from one seeded stream; spacing, h, dt and sweeps as dam_break."""
+    u = uniform01(seed * 7919 + 17, 3 * count).astype(np.float64).reshape(count, 3)
+    out = []
+    for k in range(count):
+        target = float(np.exp(np.log(n_min) + u[k, 0] * (np.log(n_max) - np.log(n_min))))
+        side = max(1, int(round(target ** (1.0 / 3.0))))
+        depth = max(1, int(round(target / (side * side))))
+        n = side * side * depth
+        edge = float((n / 32768.0) ** (1.0 / 3.0))
+        rho0 = 950.0 + 100.0 * float(u[k, 1])
+        nu = 0.005 + 0.045 * float(u[k, 2])
+        w = dam_break((side, side, depth), (edge, edge, edge), seed=seed * 1000 + k, rho0=rho0, viscosity=nu,
+                      name=f"batch{seed}_scene{k}", **kw)
+        out.append(w)
+    return out

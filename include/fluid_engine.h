@@ -211,6 +211,36 @@ fe_status fe_frame_control(const uint32_t stats[3], uint32_t n, const fe_params*
                            const fe_options* options, float* dt_out,
                            uint32_t* density_iters, uint32_t* divergence_iters);
 
+/* ---- batched scenes (no reference counterpart) -----------------------------------------------
+ * Many independent scenes of uneven sizes in one context, advanced together: every kernel of
+ * a frame is one launch over all scenes. Each scene keeps its own parameters (mass, rest
+ * density, viscosity, box, h), its own T_s = next_power_of_two(N_s)-entry hash table, its
+ * own convergence (a converged scene's particles skip the remaining sweeps) and its own stats,
+ * and evolves exactly as the same scene would in its own context (bit for bit). The scenes
+ * share dt and the fixed sweep counts (they advance in lockstep) and run the radix sort
+ * without the early-exit / CFL flags and with the reference warm start. fe_step, fe_run,
+ * fe_set_options, the per-ComputeStep calls (except fe_sort) and whole-field transfers of
+ * particle fields work on a batch context; per-scene transfers use fe_batch_*. */
+fe_status fe_batch_create(const fe_params* params, const float* const* xyz, const uint32_t* counts,
+                          uint32_t scenes, int device, fe_ctx** out);
+fe_status fe_batch_scenes(const fe_ctx* ctx, uint32_t* scenes);
+fe_status fe_batch_scene_info(const fe_ctx* ctx, uint32_t scene, uint32_t* n, uint32_t* table_size);
+/* Per-scene parameters (dt and sweep counts must stay those of the batch). */
+fe_status fe_batch_set_params(fe_ctx* ctx, uint32_t scene, const fe_params* params);
+/* One scene's field as its own context would hold it: particle fields are the scene's N_s
+ * rows; GRID_ENTRIES / GRID_START are the scene's own T_s-entry table with indices relative
+ * to the scene. Uploads take particle fields only. */
+fe_status fe_batch_download(fe_ctx* ctx, uint32_t scene, int field, void* dst, size_t bytes);
+fe_status fe_batch_upload(fe_ctx* ctx, uint32_t scene, int field, const void* src, size_t bytes);
+/* The scene's stats_buffer after the last frame. */
+fe_status fe_batch_read_stats(fe_ctx* ctx, uint32_t scene, uint32_t out[3]);
+/* The scene's sweeps in the last frame and since creation. */
+fe_status fe_batch_iterations(fe_ctx* ctx, uint32_t scene, uint32_t* density, uint32_t* divergence,
+                              uint64_t* density_total, uint64_t* divergence_total);
+/* With FE_SOLVER_FIXED, capture the steady-state frame of the batch once in a HIP graph and
+ * replay it for every later frame (parameter, option and field changes drop the graph). */
+fe_status fe_batch_use_graph(fe_ctx* ctx, int enable);
+
 /* ---- learned warm start (no reference counterpart) -----------------------------------------
  * Two networks, FE_NET_DENSITY and FE_NET_DIVERGENCE, one per Jacobi solve. Each maps the
  * FE_MLP_INPUTS features of a particle through three GELU layers of FE_MLP_HIDDEN units to
