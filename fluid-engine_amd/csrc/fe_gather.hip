@@ -226,6 +226,7 @@ __device__ void build_group(const Sph& s, const uint2* __restrict__ range, const
     }
 }
 
+template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_build_list(Sph s0, Scenes sc, uint32_t n, const uint2* __restrict__ range,
                                                        const float4* __restrict__ pos_b,
                                                        const float4* __restrict__ pos_m,
@@ -238,16 +239,16 @@ __global__ __launch_bounds__(kBlock) void k_build_list(Sph s0, Scenes sc, uint32
     const uint32_t t = base + lane;
     const bool valid = t < n;
     // a group is a run of work-items of one scene whose particles share a cell
-    const uint32_t sid = scene_id(sc, valid ? t : base);
-    const float inv_h = scene_at(sc, s0, valid ? t : base).inv_h;
+    const uint32_t sid = kBatch ? sc.sid[valid ? t : base] : 0u;
+    const float inv_h = kBatch ? sc.tab[sid].inv_h : s0.inv_h;
     const float4 p = valid ? pos_m[t] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int3 c = cell_of(p.x, p.y, p.z, inv_h);
     int3 cp = make_int3(__shfl_up(c.x, 1, 64), __shfl_up(c.y, 1, 64), __shfl_up(c.z, 1, 64));
     uint32_t sp = __shfl_up(sid, 1, 64);
     if (lane == 0 && base > 0) {
         const float4 q = pos_m[base - 1];
-        sp = scene_id(sc, base - 1);
-        cp = cell_of(q.x, q.y, q.z, scene_at(sc, s0, base - 1).inv_h);
+        sp = kBatch ? sc.sid[base - 1] : 0u;
+        cp = cell_of(q.x, q.y, q.z, kBatch ? sc.tab[sp].inv_h : s0.inv_h);
     }
     const bool start = valid && (t == 0 || c.x != cp.x || c.y != cp.y || c.z != cp.z || sid != sp);
     uint64_t starts = __ballot(start);
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_build_list(Sph s0, Scenes sc, uint32
         const uint32_t gs = base + (uint32_t)sl;
         const int3 gc = make_int3(__shfl(c.x, sl, 64), __shfl(c.y, sl, 64), __shfl(c.z, sl, 64));
         const uint32_t gsid = __shfl(sid, sl, 64);
-        const Sph s = scene_at(sc, s0, gs);
+        const Sph s = kBatch ? sc.tab[__builtin_amdgcn_readfirstlane(gsid)] : s0;
         uint32_t ge;
         if (starts) {
             ge = base + (uint32_t)__builtin_ctzll(starts);
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_build_list(Sph s0, Scenes sc, uint32
                 const float4 q = vj ? pos_m[j] : make_float4(0.f, 0.f, 0.f, 0.f);
                 const int3 cj = cell_of(q.x, q.y, q.z, s.inv_h);
                 const uint64_t diff = __ballot(!vj || cj.x != gc.x || cj.y != gc.y || cj.z != gc.z ||
-                                               scene_id(sc, vj ? j : 0) != gsid);
+                                               (kBatch && sc.sid[vj ? j : 0] != gsid));
                 if (diff) {
                     ge += (uint32_t)__builtin_ctzll(diff);
                     break;
@@ -646,7 +647,50 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v
 // bucket walk when the list is stale or the particle overflowed it. With `prev` set, a
 // converged previous sweep skips the launch; with `partial` set, each block stores the sum
 // of its particles' finish() values.
+// One work-item's gather: the listed pairs of schedule position t (or the bucket walk).
 template <class P>
+__device__ __forceinline__ unsigned long long gather_one(const P& pol, const Sph& s, uint32_t t,
+                                                         const uint2* __restrict__ range,
+                                                         const float4* __restrict__ pos_b,
+                                                         const float4* __restrict__ pos_m,
+                                                         const uint32_t* __restrict__ sched,
+                                                         const uint32_t* __restrict__ inv,
+                                                         const uint32_t* __restrict__ nbr,
+                                                         const uint32_t* __restrict__ cnt, uint32_t list_ok) {
+    const uint32_t i = sched[t];
+    const float4 pi = pos_m[t];
+    const typename P::Self si = pol.self(i, t);
+    typename P::Acc acc = pol.zero();
+    const uint32_t m = list_ok ? cnt[t] : 0xFFFFFFFFu;
+    if (m <= kMaxNbr) {
+        const uint4* tile = list_tile(nbr, t);
+        for (uint32_t k = 0; k < m; k += 4) {
+            const uint4 q4 = tile[(k >> 2) * 16];
+            const uint32_t jt[4] = {q4.x, k + 1 < m ? q4.y : q4.x, k + 2 < m ? q4.z : q4.x, k + 3 < m ? q4.w : q4.x};
+            float4 pj[4];
+            typename P::Cand cj[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                pj[u] = pos_m[jt[u]];
+                cj[u] = pol.load(jt[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (k + u < m) pol.pair(acc, si, s, pi.x - pj[u].x, pi.y - pj[u].y, pi.z - pj[u].z, cj[u]);
+        }
+    } else {
+        visit27(range, pos_b, s, pi.x, pi.y, pi.z, [&](uint32_t j, const float4& pj) {
+            pol.pair(acc, si, s, pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, pol.load(inv[j]));
+        });
+    }
+    return pol.finish(i, t, acc, si, s);
+}
+
+// One launch of a list consumer over all work-items. One scene (kBatch false): the scene's
+// constants are kernel arguments, and the sweep returns at once when the previous sweep
+// converged. A batch: each work-item takes its scene's constants from the table and skips
+// when its own scene converged (a batch's slots are kMaxSolveIters apart per scene).
+template <class P, bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_list_gather(P pol, Sph s0, Scenes sc, uint32_t n,
                                                         const uint2* __restrict__ range,
                                                         const float4* __restrict__ pos_b,
@@ -659,54 +703,25 @@ __global__ __launch_bounds__(kBlock) void k_list_gather(P pol, Sph s0, Scenes sc
                                                         const unsigned long long* __restrict__ prev,
                                                         unsigned long long thr, uint32_t kind,
                                                         unsigned long long* __restrict__ slot_out) {
-    // one scene: the sweep returns at once when the previous sweep converged; a batch: each
-    // particle skips when its own scene converged (slots are kMaxSolveIters apart per scene)
-    if (!sc.tab && prev && *prev <= thr) return;
+    if (!kBatch && prev && *prev <= thr) return;
     const uint32_t t = work_item();
     unsigned long long q = 0;
     uint32_t scene = 0;
-    bool act = t < n;
-    Sph s = s0;
-    if (act && sc.tab) {
-        scene = sc.sid[t];
-        s = sc.tab[scene];
-        if (prev && prev[(size_t)scene * kMaxSolveIters] <= s.thr[kind]) act = false;
-    }
-    if (act) {
-        const uint32_t i = sched[t];
-        const float4 pi = pos_m[t];
-        const typename P::Self si = pol.self(i, t);
-        typename P::Acc acc = pol.zero();
-        const uint32_t m = list_ok ? cnt[t] : 0xFFFFFFFFu;
-        if (m <= kMaxNbr) {
-            const uint4* tile = list_tile(nbr, t);
-            for (uint32_t k = 0; k < m; k += 4) {
-                const uint4 q4 = tile[(k >> 2) * 16];
-                const uint32_t jt[4] = {q4.x, k + 1 < m ? q4.y : q4.x, k + 2 < m ? q4.z : q4.x,
-                                        k + 3 < m ? q4.w : q4.x};
-                float4 pj[4];
-                typename P::Cand cj[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    pj[u] = pos_m[jt[u]];
-                    cj[u] = pol.load(jt[u]);
-                }
-#pragma unroll
-                for (int u = 0; u < 4; u++)
-                    if (k + u < m) pol.pair(acc, si, s, pi.x - pj[u].x, pi.y - pj[u].y, pi.z - pj[u].z, cj[u]);
-            }
+    if (t < n) {
+        if constexpr (kBatch) {
+            scene = sc.sid[t];
+            const Sph s = sc.tab[scene];
+            if (!(prev && prev[(size_t)scene * kMaxSolveIters] <= s.thr[kind]))
+                q = gather_one(pol, s, t, range, pos_b, pos_m, sched, inv, nbr, cnt, list_ok);
         } else {
-            visit27(range, pos_b, s, pi.x, pi.y, pi.z, [&](uint32_t j, const float4& pj) {
-                pol.pair(acc, si, s, pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, pol.load(inv[j]));
-            });
+            q = gather_one(pol, s0, t, range, pos_b, pos_m, sched, inv, nbr, cnt, list_ok);
         }
-        q = pol.finish(i, t, acc, si, s);
     }
     if (partial) {
         const unsigned long long b = block_sum_u64(q);
         if (threadIdx.x == 0) partial[blockIdx.x] = b;
     }
-    if (slot_out) {
+    if (kBatch && slot_out) {
         // per-scene residual sums by atomics (integer, so order-independent): one per wave
         // unless the wave straddles two scenes
         const uint32_t s_first = __shfl(scene, 0, 64);
@@ -760,9 +775,14 @@ hipError_t run_list(fe_ctx* c, const char* name, const P& pol, unsigned long lon
                     const unsigned long long* prev = nullptr, unsigned long long thr = 0, uint32_t kind = 0,
                     unsigned long long* slot_out = nullptr) {
     Launch L(c, name);
-    hipLaunchKernelGGL(k_list_gather<P>, grid_for(c->n), dim3(kBlock), 0, c->stream, pol, sph_of(c), scenes_of(c),
-                       c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr, c->nbr_count,
-                       c->list_valid ? 1u : 0u, partial, prev, thr, kind, slot_out);
+    if (c->nscenes > 1)
+        hipLaunchKernelGGL((k_list_gather<P, true>), grid_for(c->n), dim3(kBlock), 0, c->stream, pol, sph_of(c),
+                           scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr,
+                           c->nbr_count, c->list_valid ? 1u : 0u, partial, prev, thr, kind, slot_out);
+    else
+        hipLaunchKernelGGL((k_list_gather<P, false>), grid_for(c->n), dim3(kBlock), 0, c->stream, pol, sph_of(c),
+                           scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr,
+                           c->nbr_count, c->list_valid ? 1u : 0u, partial, prev, thr, kind, slot_out);
     return hipGetLastError();
 }
 
@@ -792,8 +812,12 @@ hipError_t launch_density_alpha(fe_ctx* c) {
     if (e != hipSuccess) return e;
     {
         Launch L(c, "build_list");
-        hipLaunchKernelGGL(k_build_list, grid_for(c->n), dim3(kBlock), 0, c->stream, sph_of(c), scenes_of(c), c->n,
-                           c->range, c->pos_b, c->pos_m, c->inv, c->nbr, c->nbr_count);
+        if (c->nscenes > 1)
+            hipLaunchKernelGGL(k_build_list<true>, grid_for(c->n), dim3(kBlock), 0, c->stream, sph_of(c),
+                               scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->inv, c->nbr, c->nbr_count);
+        else
+            hipLaunchKernelGGL(k_build_list<false>, grid_for(c->n), dim3(kBlock), 0, c->stream, sph_of(c),
+                               scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->inv, c->nbr, c->nbr_count);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
