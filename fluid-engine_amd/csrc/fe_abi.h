@@ -31,6 +31,9 @@ struct DeviceGuard {
     }
 };
 
+// Drop a captured frame graph (its kernel arguments are stale after any change).
+void drop_graph(fe_ctx* c);
+
 // Warm-start hooks of the Jacobi solves (fe_learn_api.cpp): before the first sweep
 // (features, prediction) and after the last (samples, the pressures kept for the next solve).
 fe_status learn_before_solve(fe_ctx* c, int kind);

@@ -136,14 +136,19 @@ fe_status upload_scene_tab(fe_ctx* c) {
     return FE_OK;
 }
 
+}  // namespace
+
 // A captured frame holds the kernel arguments of its capture; anything that changes them
 // drops it.
-void drop_graph(fe_ctx* c) {
+void fe::drop_graph(fe_ctx* c) {
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     c->graph_exec = nullptr;
     c->graph = nullptr;
 }
+
+namespace {
+using fe::drop_graph;
 
 // Frame control of Renderer::step (src/renderer/mod.rs:231-267).
 void frame_control(const uint32_t stats[3], uint32_t n, fe_params* p, const fe_options* o, uint32_t* dit,

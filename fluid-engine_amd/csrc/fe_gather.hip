@@ -37,7 +37,8 @@ __device__ __forceinline__ uint32_t xcd_block() {
     return x * per + (x < rem ? x : rem) + k;
 }
 
-__device__ __forceinline__ uint32_t work_item() { return xcd_block() * kBlock + threadIdx.x; }
+template <int B>
+__device__ __forceinline__ uint32_t work_item() { return xcd_block() * B + threadIdx.x; }
 
 // The 27-bucket walk in reference order over the hash-ordered positions. The nine ranges
 // of one z slab are loaded together and candidates are fetched four at a time.
@@ -132,7 +133,6 @@ __device__ __forceinline__ size_t list_at(uint32_t t, uint32_t slot) {
     return ((((size_t)(t >> 4) * (kMaxNbr / 4) + (slot >> 2)) << 6) + ((t & 15u) << 2) + (slot & 3u));
 }
 
-constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kCand = 256;   // candidates staged per pass per wave
 
 struct BuildStage {
@@ -226,8 +226,8 @@ __device__ void build_group(const Sph& s, const uint2* __restrict__ range, const
     }
 }
 
-template <bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_build_list(Sph s0, Scenes sc, uint32_t n, const uint2* __restrict__ range,
+template <bool kBatch, int kWaves>
+__global__ __launch_bounds__(64 * kWaves) void k_build_list(Sph s0, Scenes sc, uint32_t n, const uint2* __restrict__ range,
                                                        const float4* __restrict__ pos_b,
                                                        const float4* __restrict__ pos_m,
                                                        const uint32_t* __restrict__ inv, uint32_t* __restrict__ nbr,
@@ -630,8 +630,9 @@ struct PressureUpdate {
     }
 };
 
+template <int B>
 __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v) {
-    __shared__ unsigned long long part[kBlock / 64];
+    __shared__ unsigned long long part[B / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -639,7 +640,7 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v
     __syncthreads();
     unsigned long long t = 0;
     if (threadIdx.x == 0)
-        for (int k = 0; k < (int)(kBlock / 64); k++) t += part[k];
+        for (int k = 0; k < B / 64; k++) t += part[k];
     return t;
 }
 
@@ -690,8 +691,8 @@ __device__ __forceinline__ unsigned long long gather_one(const P& pol, const Sph
 // constants are kernel arguments, and the sweep returns at once when the previous sweep
 // converged. A batch: each work-item takes its scene's constants from the table and skips
 // when its own scene converged (a batch's slots are kMaxSolveIters apart per scene).
-template <class P, bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_list_gather(P pol, Sph s0, Scenes sc, uint32_t n,
+template <class P, bool kBatch, int B>
+__global__ __launch_bounds__(B) void k_list_gather(P pol, Sph s0, Scenes sc, uint32_t n,
                                                         const uint2* __restrict__ range,
                                                         const float4* __restrict__ pos_b,
                                                         const float4* __restrict__ pos_m,
@@ -704,7 +705,7 @@ __global__ __launch_bounds__(kBlock) void k_list_gather(P pol, Sph s0, Scenes sc
                                                         unsigned long long thr, uint32_t kind,
                                                         unsigned long long* __restrict__ slot_out) {
     if (!kBatch && prev && *prev <= thr) return;
-    const uint32_t t = work_item();
+    const uint32_t t = work_item<B>();
     unsigned long long q = 0;
     uint32_t scene = 0;
     if (t < n) {
@@ -718,7 +719,7 @@ __global__ __launch_bounds__(kBlock) void k_list_gather(P pol, Sph s0, Scenes sc
         }
     }
     if (partial) {
-        const unsigned long long b = block_sum_u64(q);
+        const unsigned long long b = block_sum_u64<B>(q);
         if (threadIdx.x == 0) partial[blockIdx.x] = b;
     }
     if (kBatch && slot_out) {
@@ -770,25 +771,44 @@ hipError_t need(fe_ctx* c, uint32_t bits) {
     return hipGetLastError();
 }
 
+template <class P, bool kBatch, int B>
+void launch_gather(fe_ctx* c, const P& pol, unsigned long long* partial, const unsigned long long* prev,
+                   unsigned long long thr, uint32_t kind, unsigned long long* slot_out) {
+    hipLaunchKernelGGL((k_list_gather<P, kBatch, B>), dim3((c->n + B - 1) / B), dim3(B), 0, c->stream, pol,
+                       sph_of(c), scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr,
+                       c->nbr_count, c->list_valid ? 1u : 0u, partial, prev, thr, kind, slot_out);
+}
+
+template <class P, bool kBatch>
+void launch_gather_b(fe_ctx* c, uint32_t block, const P& pol, unsigned long long* partial,
+                     const unsigned long long* prev, unsigned long long thr, uint32_t kind,
+                     unsigned long long* slot_out) {
+    switch (block) {
+        case 64: launch_gather<P, kBatch, 64>(c, pol, partial, prev, thr, kind, slot_out); break;
+        case 128: launch_gather<P, kBatch, 128>(c, pol, partial, prev, thr, kind, slot_out); break;
+        case 512: launch_gather<P, kBatch, 512>(c, pol, partial, prev, thr, kind, slot_out); break;
+        case 1024: launch_gather<P, kBatch, 1024>(c, pol, partial, prev, thr, kind, slot_out); break;
+        default: launch_gather<P, kBatch, 256>(c, pol, partial, prev, thr, kind, slot_out); break;
+    }
+}
+
 template <class P>
-hipError_t run_list(fe_ctx* c, const char* name, const P& pol, unsigned long long* partial = nullptr,
+hipError_t run_list(fe_ctx* c, const char* name, int tid, const P& pol, unsigned long long* partial = nullptr,
                     const unsigned long long* prev = nullptr, unsigned long long thr = 0, uint32_t kind = 0,
                     unsigned long long* slot_out = nullptr) {
     Launch L(c, name);
+    const uint32_t block = c->tune[tid].block;
     if (c->nscenes > 1)
-        hipLaunchKernelGGL((k_list_gather<P, true>), grid_for(c->n), dim3(kBlock), 0, c->stream, pol, sph_of(c),
-                           scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr,
-                           c->nbr_count, c->list_valid ? 1u : 0u, partial, prev, thr, kind, slot_out);
+        launch_gather_b<P, true>(c, block, pol, partial, prev, thr, kind, slot_out);
     else
-        hipLaunchKernelGGL((k_list_gather<P, false>), grid_for(c->n), dim3(kBlock), 0, c->stream, pol, sph_of(c),
-                           scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr,
-                           c->nbr_count, c->list_valid ? 1u : 0u, partial, prev, thr, kind, slot_out);
+        launch_gather_b<P, false>(c, block, pol, partial, prev, thr, kind, slot_out);
     return hipGetLastError();
 }
 
 }  // namespace
 
-uint32_t gather_partials(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+// Residual partials: one per block of the smallest gather block size.
+uint32_t gather_partials(uint32_t n) { return (n + 63) / 64; }
 
 hipError_t launch_schedule_mirrors(fe_ctx* c) {
     Launch L(c, "schedule_mirrors");
@@ -807,33 +827,46 @@ hipError_t launch_reorder_velocity(fe_ctx* c) {
     return hipGetLastError();
 }
 
-hipError_t launch_density_alpha(fe_ctx* c) {
+hipError_t launch_build_list(fe_ctx* c) {
     hipError_t e = need(c, kMirrorPos);   // positions by schedule position + inverse schedule
     if (e != hipSuccess) return e;
-    {
-        Launch L(c, "build_list");
-        if (c->nscenes > 1)
-            hipLaunchKernelGGL(k_build_list<true>, grid_for(c->n), dim3(kBlock), 0, c->stream, sph_of(c),
-                               scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->inv, c->nbr, c->nbr_count);
-        else
-            hipLaunchKernelGGL(k_build_list<false>, grid_for(c->n), dim3(kBlock), 0, c->stream, sph_of(c),
-                               scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->inv, c->nbr, c->nbr_count);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
+    Launch L(c, "build_list");
+    const uint32_t waves = c->tune[kTuneBuildList].block / 64;
+    const dim3 grid((c->n + 64 * waves - 1) / (64 * waves)), blk(64 * waves);
+    const Sph s0 = sph_of(c);
+    const Scenes sc = scenes_of(c);
+#define FE_BUILD(BATCH, W)                                                                                   \
+    hipLaunchKernelGGL((k_build_list<BATCH, W>), grid, blk, 0, c->stream, s0, sc, c->n, c->range, c->pos_b, \
+                       c->pos_m, c->inv, c->nbr, c->nbr_count)
+    if (c->nscenes > 1) {
+        if (waves == 1) FE_BUILD(true, 1); else if (waves == 2) FE_BUILD(true, 2); else FE_BUILD(true, 4);
+    } else {
+        if (waves == 1) FE_BUILD(false, 1); else if (waves == 2) FE_BUILD(false, 2); else FE_BUILD(false, 4);
     }
-    c->list_valid = true;
+#undef FE_BUILD
+    e = hipGetLastError();
+    if (e == hipSuccess) c->list_valid = true;
+    return e;
+}
+
+hipError_t launch_density_alpha_only(fe_ctx* c) {
     DensityAlpha pol{c->dens, c->fac, c->dens_m};
-    e = run_list(c, "density_alpha", pol);
+    hipError_t e = run_list(c, "density_alpha", kTuneDensityAlpha, pol);
     c->mirrors = (c->mirrors | kMirrorDens) & ~kMirrorPrho;   // p / rho^2 used the old densities
     c->da_fresh = c->ns_fresh;
     return e;
+}
+
+hipError_t launch_density_alpha(fe_ctx* c) {
+    hipError_t e = launch_build_list(c);
+    return e == hipSuccess ? launch_density_alpha_only(c) : e;
 }
 
 hipError_t launch_viscosity(fe_ctx* c) {
     hipError_t e = need(c, kMirrorPos | kMirrorVelB | kMirrorDens);
     if (e != hipSuccess) return e;
     Viscosity pol{c->vb_m, c->dens_m, c->vel_a, c->va_m};
-    e = run_list(c, "viscosity", pol);
+    e = run_list(c, "viscosity", kTuneViscosity, pol);
     c->mirrors |= kMirrorVelA;
     return e;
 }
@@ -845,10 +878,10 @@ hipError_t launch_density_source(fe_ctx* c) {
     if (learn_features_wanted(c, 0)) {
         if ((e = learn_prev(c, 0)) != hipSuccess) return e;
         DensitySourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n};
-        e = run_list(c, "density_source_feat", fp);
+        e = run_list(c, "density_source_feat", kTuneDensitySource, fp);
         learn_features_done(c, 0);
     } else {
-        e = run_list(c, "density_source", pol);
+        e = run_list(c, "density_source", kTuneDensitySource, pol);
     }
     c->mirrors |= kMirrorPrho;
     return e;
@@ -861,10 +894,10 @@ hipError_t launch_divergence_source(fe_ctx* c) {
     if (learn_features_wanted(c, 1)) {
         if ((e = learn_prev(c, 1)) != hipSuccess) return e;
         DivergenceSourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n};
-        e = run_list(c, "divergence_source_feat", fp);
+        e = run_list(c, "divergence_source_feat", kTuneDivergenceSource, fp);
         learn_features_done(c, 1);
     } else {
-        e = run_list(c, "divergence_source", pol);
+        e = run_list(c, "divergence_source", kTuneDivergenceSource, pol);
     }
     c->mirrors |= kMirrorPrho;
     return e;
@@ -874,7 +907,7 @@ hipError_t launch_pressure_force(fe_ctx* c, const unsigned long long* prev, unsi
     hipError_t e = need(c, kMirrorPos | kMirrorPrho);
     if (e != hipSuccess) return e;
     PressureForce pol{c->pres, c->dens, c->prho_m, c->pacc, c->pacc_m};
-    e = run_list(c, "pressure_force", pol, nullptr, prev, thr, kind);
+    e = run_list(c, "pressure_force", kTunePressureForce, pol, nullptr, prev, thr, kind);
     c->mirrors |= kMirrorPacc;
     return e;
 }
@@ -885,11 +918,12 @@ hipError_t launch_pressure_update(fe_ctx* c, unsigned long long* out, const unsi
     if (e != hipSuccess) return e;
     PressureUpdate pol{c->pacc_m, c->fac, c->src, c->dens, c->pres, c->prho_m};
     if (c->nscenes > 1)   // per-scene sums straight into the scenes' slots
-        return run_list(c, "pressure_update", pol, nullptr, prev, thr, kind, out);
-    e = run_list(c, "pressure_update", pol, out ? c->partial : nullptr, prev, thr, kind);
+        return run_list(c, "pressure_update", kTunePressureUpdate, pol, nullptr, prev, thr, kind, out);
+    e = run_list(c, "pressure_update", kTunePressureUpdate, pol, out ? c->partial : nullptr, prev, thr, kind);
     if (e != hipSuccess || !out) return e;
     Launch L(c, "resid_finalize");
-    hipLaunchKernelGGL(k_resid_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->partial, gather_partials(c->n), out,
+    const uint32_t pb = c->tune[kTunePressureUpdate].block;
+    hipLaunchKernelGGL(k_resid_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->partial, (c->n + pb - 1) / pb, out,
                        prev, thr);
     return hipGetLastError();
 }

@@ -94,6 +94,28 @@ struct LearnState {
 struct Launch;
 struct Sph;
 
+// Launch configurations the autotuner chooses (block = threads per block, items = particles
+// per thread for the streaming kernels). Results never depend on them.
+enum TuneId {
+    kTuneDensityAlpha,
+    kTuneViscosity,
+    kTuneDensitySource,
+    kTuneDivergenceSource,
+    kTunePressureForce,
+    kTunePressureUpdate,
+    kTuneBuildList,
+    kTunePressureIntegration,
+    kTuneDivergenceIntegration,
+    kTuneCount
+};
+struct TuneCfg {
+    uint32_t block = 256, items = 1;
+    float median_us = 0.0f, default_us = 0.0f;   // of the last fe_autotune
+};
+const char* tune_name(int id);
+// Whether (block, items) is a configuration kernel `id` is built for.
+bool tune_valid(int id, uint32_t block, uint32_t items);
+
 }  // namespace fe
 
 // The context: SOA particle state in HBM, laid out as GpuPhysicsData
@@ -169,6 +191,7 @@ struct fe_ctx {
 
     fe::SortScratch sort;
     fe::LearnState ws;
+    fe::TuneCfg tune[fe::kTuneCount];
 
     // stats_buffer and the convergence residuals (one slot per sweep, two solves)
     uint32_t* stats = nullptr;
@@ -237,7 +260,9 @@ hipError_t launch_iota(fe_ctx* c, uint32_t* p, uint32_t count);
 hipError_t launch_fill_ranges(fe_ctx* c);
 hipError_t launch_offsets_only(fe_ctx* c);
 hipError_t launch_rebuild_ends(fe_ctx* c);
-hipError_t launch_density_alpha(fe_ctx* c);
+hipError_t launch_density_alpha(fe_ctx* c);   // build_list, then the density_alpha gather
+hipError_t launch_build_list(fe_ctx* c);
+hipError_t launch_density_alpha_only(fe_ctx* c);
 hipError_t launch_viscosity(fe_ctx* c);
 hipError_t launch_density_source(fe_ctx* c);
 // Jacobi sweep kernels. prev: the previous sweep's residual slot (nullptr: no early exit);

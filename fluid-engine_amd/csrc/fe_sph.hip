@@ -127,11 +127,10 @@ __global__ __launch_bounds__(kBlock) void k_rebuild_ends(const uint32_t* __restr
 // ---- streaming stages ----------------------------------------------------------------
 
 // pressure_integration.comp:16-77 (position_b -> position_a; collision order y, x, z).
-__global__ __launch_bounds__(kBlock) void k_pressure_integration(
-    uint32_t n, Sph s0, Scenes sc, const float4* __restrict__ pos_b, const float4* __restrict__ pacc,
-    float4* __restrict__ vel, float4* __restrict__ pos_a) {
-    uint32_t i = gid();
-    if (i >= n) return;
+__device__ __forceinline__ void pressure_integration_one(uint32_t i, const Sph& s0, const Scenes& sc,
+                                                         const float4* __restrict__ pos_b,
+                                                         const float4* __restrict__ pacc, float4* __restrict__ vel,
+                                                         float4* __restrict__ pos_a) {
     const Sph s = scene_at(sc, s0, i);
     const float dt = s.dt, rad = s.rad;
     const float3 mn = make_float3(s.mnx, s.mny, s.mnz), mx = make_float3(s.mxx, s.mxy, s.mxz);
@@ -158,15 +157,25 @@ __global__ __launch_bounds__(kBlock) void k_pressure_integration(
     vel[i] = make_float4(vx, vy, vz, 0.0f);
 }
 
+// B threads per block, IPT particles per thread (block-contiguous, so each round coalesces).
+template <int B, int IPT>
+__global__ __launch_bounds__(B) void k_pressure_integration(uint32_t n, Sph s0, Scenes sc,
+                                                            const float4* __restrict__ pos_b,
+                                                            const float4* __restrict__ pacc, float4* __restrict__ vel,
+                                                            float4* __restrict__ pos_a) {
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const uint32_t i = (blockIdx.x * IPT + k) * B + threadIdx.x;
+        if (i < n) pressure_integration_one(i, s0, sc, pos_b, pacc, vel, pos_a);
+    }
+}
+
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
 
 // divergence_integration.comp:14-43.
-__global__ __launch_bounds__(kBlock) void k_divergence_integration(uint32_t n, Sph s0, Scenes sc,
-                                                                   float4* __restrict__ vel,
-                                                                   const float4* __restrict__ pacc,
-                                                                   float4* __restrict__ colors) {
-    uint32_t i = gid();
-    if (i >= n) return;
+__device__ __forceinline__ void divergence_integration_one(uint32_t i, const Sph& s0, const Scenes& sc,
+                                                           float4* __restrict__ vel, const float4* __restrict__ pacc,
+                                                           float4* __restrict__ colors) {
     const float dt = scene_at(sc, s0, i).dt;
     const float4 v = vel[i], a = pacc[i];
     float vx = v.x + a.x * dt, vy = v.y + a.y * dt, vz = v.z + a.z * dt;
@@ -184,6 +193,41 @@ __global__ __launch_bounds__(kBlock) void k_divergence_integration(uint32_t n, S
     vel[i] = make_float4(vx, vy, vz, 0.0f);
     colors[i] = make_float4(c0, c1, c2, 1.0f);
 }
+
+template <int B, int IPT>
+__global__ __launch_bounds__(B) void k_divergence_integration(uint32_t n, Sph s0, Scenes sc, float4* __restrict__ vel,
+                                                              const float4* __restrict__ pacc,
+                                                              float4* __restrict__ colors) {
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const uint32_t i = (blockIdx.x * IPT + k) * B + threadIdx.x;
+        if (i < n) divergence_integration_one(i, s0, sc, vel, pacc, colors);
+    }
+}
+
+// Dispatch of a streaming kernel template over the tuned (block, items) configurations.
+#define FE_STREAM_DISPATCH(KERNEL, cfg, n, ...)                                                          \
+    do {                                                                                                 \
+        const uint32_t b_ = (cfg).block, it_ = (cfg).items;                                              \
+        const dim3 g_((unsigned)(((n) + b_ * it_ - 1) / (b_ * it_)));                                     \
+        switch (b_ * 8 + it_) {                                                                          \
+            case 64 * 8 + 1: hipLaunchKernelGGL((KERNEL<64, 1>), g_, dim3(64), 0, __VA_ARGS__); break;     \
+            case 64 * 8 + 2: hipLaunchKernelGGL((KERNEL<64, 2>), g_, dim3(64), 0, __VA_ARGS__); break;     \
+            case 64 * 8 + 4: hipLaunchKernelGGL((KERNEL<64, 4>), g_, dim3(64), 0, __VA_ARGS__); break;     \
+            case 128 * 8 + 1: hipLaunchKernelGGL((KERNEL<128, 1>), g_, dim3(128), 0, __VA_ARGS__); break;  \
+            case 128 * 8 + 2: hipLaunchKernelGGL((KERNEL<128, 2>), g_, dim3(128), 0, __VA_ARGS__); break;  \
+            case 128 * 8 + 4: hipLaunchKernelGGL((KERNEL<128, 4>), g_, dim3(128), 0, __VA_ARGS__); break;  \
+            case 256 * 8 + 2: hipLaunchKernelGGL((KERNEL<256, 2>), g_, dim3(256), 0, __VA_ARGS__); break;  \
+            case 256 * 8 + 4: hipLaunchKernelGGL((KERNEL<256, 4>), g_, dim3(256), 0, __VA_ARGS__); break;  \
+            case 512 * 8 + 1: hipLaunchKernelGGL((KERNEL<512, 1>), g_, dim3(512), 0, __VA_ARGS__); break;  \
+            case 512 * 8 + 2: hipLaunchKernelGGL((KERNEL<512, 2>), g_, dim3(512), 0, __VA_ARGS__); break;  \
+            case 512 * 8 + 4: hipLaunchKernelGGL((KERNEL<512, 4>), g_, dim3(512), 0, __VA_ARGS__); break;  \
+            case 1024 * 8 + 1: hipLaunchKernelGGL((KERNEL<1024, 1>), g_, dim3(1024), 0, __VA_ARGS__); break; \
+            case 1024 * 8 + 2: hipLaunchKernelGGL((KERNEL<1024, 2>), g_, dim3(1024), 0, __VA_ARGS__); break; \
+            case 1024 * 8 + 4: hipLaunchKernelGGL((KERNEL<1024, 4>), g_, dim3(1024), 0, __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL((KERNEL<256, 1>), g_, dim3(256), 0, __VA_ARGS__); break;           \
+        }                                                                                                \
+    } while (0)
 
 __device__ __forceinline__ uint32_t to_u32_sat(float x) {
     if (!(x > 0.0f)) return 0u;
@@ -402,7 +446,7 @@ hipError_t launch_pressure_integration(fe_ctx* c) {
     c->ns_fresh = false;   // position_a moves
     c->da_fresh = false;
     Launch L(c, "pressure_integration");
-    hipLaunchKernelGGL(k_pressure_integration, grid_for(c->n), dim3(kBlock), 0, c->stream, c->n, sph_of(c),
+    FE_STREAM_DISPATCH(k_pressure_integration, c->tune[kTunePressureIntegration], c->n, c->stream, c->n, sph_of(c),
                        scenes_of(c), c->pos_b, c->pacc, c->vel_a, c->pos_a);
     return hipGetLastError();
 }
@@ -411,8 +455,8 @@ hipError_t launch_pressure_integration(fe_ctx* c) {
 hipError_t launch_divergence_integration(fe_ctx* c) {
     c->mirrors &= ~kMirrorVelA;
     Launch L(c, "divergence_integration");
-    hipLaunchKernelGGL(k_divergence_integration, grid_for(c->n), dim3(kBlock), 0, c->stream, c->n, sph_of(c),
-                       scenes_of(c), c->vel_a, c->pacc, c->colors);
+    FE_STREAM_DISPATCH(k_divergence_integration, c->tune[kTuneDivergenceIntegration], c->n, c->stream, c->n,
+                       sph_of(c), scenes_of(c), c->vel_a, c->pacc, c->colors);
     return hipGetLastError();
 }
 
@@ -458,6 +502,21 @@ hipError_t launch_unpack_entries(fe_ctx* c, const uint32_t* in) {
 hipError_t launch_pack_start(fe_ctx* c, uint32_t* out) {
     hipLaunchKernelGGL(k_pack_start, grid_for(c->T), dim3(kBlock), 0, c->stream, c->range, c->T, out);
     return hipGetLastError();
+}
+
+const char* tune_name(int id) {
+    static const char* names[kTuneCount] = {"density_alpha",     "viscosity",      "density_source",
+                                            "divergence_source", "pressure_force", "pressure_update",
+                                            "build_list",        "pressure_integration", "divergence_integration"};
+    return id >= 0 && id < kTuneCount ? names[id] : "";
+}
+
+bool tune_valid(int id, uint32_t block, uint32_t items) {
+    const bool pow2 = block == 64 || block == 128 || block == 256 || block == 512 || block == 1024;
+    if (id == kTuneBuildList) return (block == 64 || block == 128 || block == 256) && items == 1;
+    if (id == kTunePressureIntegration || id == kTuneDivergenceIntegration)
+        return pow2 && (items == 1 || items == 2 || items == 4);
+    return id >= 0 && id < kTuneCount && pow2 && items == 1;
 }
 
 hipError_t launch_unpack_start(fe_ctx* c, const uint32_t* in) {

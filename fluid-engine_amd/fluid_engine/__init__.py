@@ -13,11 +13,11 @@ from ctypes import byref, c_double, c_float, c_uint32, c_uint64, c_void_p
 
 import numpy as np
 
-from . import scene
+from . import scene, tuning
 from ._ffi import (EXPORTED_SYMBOLS, FE_MLP_HIDDEN, FE_MLP_INPUTS, FE_MLP_NORM, FE_MLP_PARAMS, FE_NET_DENSITY,
                    FE_NET_DIVERGENCE, FE_SOLVER_CONVERGE, FE_SOLVER_FIXED, FE_SORT_BITONIC, FE_SORT_RADIX,
                    FE_WARM_START_LEARNED, FE_WARM_START_REFERENCE, FIELDS, INT_FIELDS, LIB_PATH, FeOptions,
-                   FeParams, FluidEngineError, check, load_library)
+                   FeParams, FeTuneEntry, FluidEngineError, check, load_library)
 
 __all__ = [
     "Simulation", "FeParams", "FeOptions", "FluidEngineError", "load_library", "scene", "frame_control",
@@ -234,6 +234,36 @@ class Simulation:
                   "fe_profile_entry")
             res[buf.value.decode()] = (launches.value, total.value)
         return res
+
+    # -- launch-configuration autotuner ---------------------------------------------------
+    def device_name(self) -> str:
+        buf = ctypes.create_string_buffer(128)
+        check(self._lib.fe_device_name(self._ctx, buf, 128), "fe_device_name")
+        return buf.value.decode()
+
+    def autotune_now(self, reps: int = 20) -> dict:
+        """Time every launch configuration on the current state and keep the fastest."""
+        check(self._lib.fe_autotune(self._ctx, reps), "fe_autotune")
+        return self.tune_entries()
+
+    def tune_entries(self) -> dict:
+        cnt = c_uint32()
+        check(self._lib.fe_tune_count(self._ctx, byref(cnt)), "fe_tune_count")
+        out = {}
+        e = FeTuneEntry()
+        for i in range(cnt.value):
+            check(self._lib.fe_tune_get(self._ctx, i, byref(e)), "fe_tune_get")
+            out[e.kernel.decode()] = {"block": e.block, "items": e.items, "median_us": round(e.median_us, 3),
+                                      "default_us": round(e.default_us, 3)}
+        return out
+
+    def set_tune(self, kernel: str, block: int, items: int = 1):
+        check(self._lib.fe_tune_set(self._ctx, kernel.encode(), block, items), "fe_tune_set")
+
+    def autotune(self, cache_path: str | None = None, reps: int = 20, retune: bool = False) -> dict:
+        """The tuned configuration for this device and workload: from the JSON cache when it
+        holds one, else measured now (and stored there)."""
+        return tuning.autotune(self, cache_path, reps, retune)
 
     # -- learned warm start (no reference counterpart) -------------------------------------
     def learn_collect(self, samples_per_solve: int, capacity: int = 0):

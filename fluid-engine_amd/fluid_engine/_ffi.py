@@ -67,6 +67,16 @@ class FeParams(Structure):
     ]
 
 
+class FeTuneEntry(Structure):
+    _fields_ = [
+        ("kernel", ctypes.c_char * 32),
+        ("block", c_uint32),
+        ("items", c_uint32),
+        ("median_us", c_float),
+        ("default_us", c_float),
+    ]
+
+
 class FeOptions(Structure):
     _fields_ = [
         ("sort_algorithm", c_uint32),
@@ -157,6 +167,11 @@ _SIGNATURES = [
     ("fe_batch_iterations", [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint64),
                              POINTER(c_uint64)]),
     ("fe_batch_use_graph", [c_void_p, c_int]),
+    ("fe_autotune", [c_void_p, c_uint32]),
+    ("fe_tune_count", [c_void_p, POINTER(c_uint32)]),
+    ("fe_tune_get", [c_void_p, c_uint32, POINTER(FeTuneEntry)]),
+    ("fe_tune_set", [c_void_p, c_char_p, c_uint32, c_uint32]),
+    ("fe_device_name", [c_void_p, c_char_p, c_size_t]),
 ]
 
 EXPORTED_SYMBOLS = tuple(name for name, _ in _SIGNATURES)

@@ -211,6 +211,29 @@ fe_status fe_frame_control(const uint32_t stats[3], uint32_t n, const fe_params*
                            const fe_options* options, float* dt_out,
                            uint32_t* density_iters, uint32_t* divergence_iters);
 
+/* ---- launch-configuration autotuner (no reference counterpart) --------------------------------
+ * Tunable kernels: the six list gathers (density_alpha, viscosity, density_source,
+ * divergence_source, pressure_force, pressure_update: block 64..1024), build_list (block
+ * 64, 128, 256) and the two integrations (block 64..1024 x 1, 2 or 4 particles per thread).
+ * fe_autotune times every configuration on the context's current state (the median of
+ * `reps` launches, 0 = 20) and keeps the fastest; the fields the timed launches overwrite
+ * are restored, so the next frame is the one the context would have run. Configurations
+ * never change results. A JSON cache keyed by device and workload is kept by the Python
+ * host (fluid_engine.tuning). */
+typedef struct fe_tune_entry {
+    char kernel[32];
+    uint32_t block;
+    uint32_t items;
+    float median_us;    /* of the chosen configuration in the last fe_autotune (0 if none) */
+    float default_us;   /* of the default configuration (block 256, 1 item) in that run */
+} fe_tune_entry;
+fe_status fe_autotune(fe_ctx* ctx, uint32_t reps);
+fe_status fe_tune_count(const fe_ctx* ctx, uint32_t* count);
+fe_status fe_tune_get(const fe_ctx* ctx, uint32_t index, fe_tune_entry* out);
+fe_status fe_tune_set(fe_ctx* ctx, const char* kernel, uint32_t block, uint32_t items);
+/* "<gcn arch>/<compute units>cu" of the context's device (the tuning cache key). */
+fe_status fe_device_name(const fe_ctx* ctx, char* name, size_t cap);
+
 /* ---- batched scenes (no reference counterpart) -----------------------------------------------
  * Many independent scenes of uneven sizes in one context, advanced together: every kernel of
  * a frame is one launch over all scenes. Each scene keeps its own parameters (mass, rest

@@ -1,0 +1,53 @@
+"""The autotuner's JSON cache (fluid_engine.tuning) with a stand-in context: a cached
+workload is applied without measuring, a new one is measured once and stored, and a
+damaged or foreign file is ignored."""
+import json
+
+from fluid_engine import tuning
+
+
+class FakeSim:
+    def __init__(self, n=1000, scenes=1):
+        self.n = n
+        self.scenes = scenes
+        self.measured = 0
+        self.applied = {}
+
+    def device_name(self):
+        return "gfx950/256cu"
+
+    def autotune_now(self, reps):
+        self.measured += 1
+        return {"pressure_force": {"block": 128, "items": 1, "median_us": 10.0, "default_us": 12.0}}
+
+    def set_tune(self, kernel, block, items):
+        self.applied[kernel] = (block, items)
+
+
+def test_cache_roundtrip(tmp_path):
+    path = str(tmp_path / "tune.json")
+    a = FakeSim()
+    r = tuning.autotune(a, path, reps=3)
+    assert a.measured == 1 and r["pressure_force"]["block"] == 128
+    data = json.load(open(path))
+    assert data["version"] == 1
+    assert tuning.cache_key("gfx950/256cu", 1000, 1) in data["entries"]
+    b = FakeSim()
+    tuning.autotune(b, path)
+    assert b.measured == 0 and b.applied == {"pressure_force": (128, 1)}
+    c = FakeSim(n=2000)                       # another workload: measured, added
+    tuning.autotune(c, path)
+    assert c.measured == 1
+    assert len(json.load(open(path))["entries"]) == 2
+    d = FakeSim()
+    tuning.autotune(d, path, retune=True)     # forced re-measure
+    assert d.measured == 1
+
+
+def test_bad_cache_is_ignored(tmp_path):
+    path = tmp_path / "tune.json"
+    path.write_text(json.dumps({"version": 99, "entries": {"x": 1}}))
+    s = FakeSim()
+    tuning.autotune(s, str(path))
+    assert s.measured == 1
+    assert json.load(open(path))["version"] == 1
