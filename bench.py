@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes measurement")
+    ap.add_argument("--no-tune", action="store_true", help="run the default launch configurations")
+    ap.add_argument("--tune-cache", default=None, help="JSON cache of tuned configurations (read and written)")
     ap.add_argument("--batch-frames", type=int, default=10)
     ap.add_argument("--learned-collect", type=int, default=100, help="frames sampled before training")
     ap.add_argument("--learned-eval", type=int, default=400, help="frames compared after training")
@@ -124,6 +126,12 @@ def main():
 
     sim.run(args.warmup, dt)
     sim.sync()
+    tuned = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tuned = sim.autotune(args.tune_cache, reps=20)
+        print(f"[bench] autotune {time.perf_counter() - t0:.2f} s: " +
+              json.dumps({k: (v["block"], v["items"]) for k, v in tuned.items()}), file=sys.stderr, flush=True)
     sim.reset_iteration_totals()
     t0 = time.perf_counter()
     sim.timer_start()
@@ -132,6 +140,17 @@ def main():
     wall = time.perf_counter() - t0
     d_it, v_it, frames = sim.iteration_totals()
     print(f"[bench] timed {args.steps} steps: {ms:.2f} ms (wall {wall * 1e3:.2f} ms)", file=sys.stderr, flush=True)
+    untuned = None
+    if tuned is not None:
+        # the same frames with the default configurations, from the same initial state
+        ref = fe.Simulation(w.params, w.positions, device=0, options=opts)
+        ref.run(args.warmup, dt)
+        ref.sync()
+        ref.timer_start()
+        ref.run(args.steps, dt)
+        ums = ref.timer_stop()
+        ref.close()
+        untuned = round(args.steps / (ums / 1e3), 3)
 
     # Per-kernel durations: every launch of a few more steps bracketed by HIP events on the
     # context's stream. These steps run the timed run's mean sweep counts as fixed sweeps, so
@@ -196,16 +215,20 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(dom),
                      "algo_bytes_per_particle": dom_bytes, "avg_launch_us": kernels[dom]["avg_us"]},
         "cpu_baseline": cpu,
+        "autotune": None if tuned is None else {
+            "untuned_steps_per_s": untuned,
+            "configs": {k: [v["block"], v["items"]] for k, v in tuned.items()},
+            "median_us_tuned_vs_default": {k: [v["median_us"], v["default_us"]] for k, v in tuned.items()}},
     }
     sim.close()
     if not args.no_learned and args.solver == "converge":
-        out["learned_warm_start"] = learned_comparison(fe, w, opts, args.learned_collect, args.learned_eval)
+        out["learned_warm_start"] = learned_comparison(fe, w, opts, args.learned_collect, args.learned_eval, tuned)
     if not args.no_batch:
         out["batched_scenes"] = batched_measurement(fe, scene, args.batch_frames)
     print(json.dumps(out), flush=True)
 
 
-def learned_comparison(fe, w, opts, collect, evaluate):
+def learned_comparison(fe, w, opts, collect, evaluate, tuned=None):
     """Learned warm start on the same workload: two contexts from the same initial state; the
     second samples 2048 (features, final pressure) pairs per solve over the first `collect`
     frames, both networks are trained on the GPU (2000 Adam steps, batch 4096), and then both
@@ -214,6 +237,9 @@ def learned_comparison(fe, w, opts, collect, evaluate):
     of each over the evaluated frames."""
     ref = fe.Simulation(w.params, w.positions, device=0, options=opts)
     lrn = fe.Simulation(w.params, w.positions, device=0, options=opts)
+    for k, v in (tuned or {}).items():   # both contexts run the tuned configurations
+        ref.set_tune(k, v["block"], v["items"])
+        lrn.set_tune(k, v["block"], v["items"])
     lrn.learn_collect(2048, 1 << 18)
     ref.run(collect)
     lrn.run(collect)
@@ -296,6 +322,10 @@ def batched_measurement(fe, scene, frames):
         return o
 
     b = fe.Batch(ws, device=0, options=opt(fe.FE_SOLVER_FIXED))
+    b.run(2)
+    b.sync()
+    tuned = b.autotune_now(reps=10)
+    res["autotuned"] = {k: [v["block"], v["items"]] for k, v in tuned.items()}
     fps = timed(b, frames)
     res["fixed_4_4_eager"] = {"frames_per_s": round(fps, 3), "particle_updates_per_s": round(fps * n_all, 1)}
     b.use_graph(True)
@@ -303,6 +333,8 @@ def batched_measurement(fe, scene, frames):
     res["fixed_4_4_graph"] = {"frames_per_s": round(fps, 3), "particle_updates_per_s": round(fps * n_all, 1)}
     b.close()
     b = fe.Batch(ws, device=0, options=opt(fe.FE_SOLVER_CONVERGE))
+    for k, (blk, it) in res["autotuned"].items():
+        b.set_tune(k, blk, it)
     fps = timed(b, frames)
     its = [b.scene_iterations(k)[1][0] for k in range(len(ws))]
     res["converge_eager"] = {"frames_per_s": round(fps, 3), "particle_updates_per_s": round(fps * n_all, 1),
