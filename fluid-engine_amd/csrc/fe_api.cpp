@@ -63,7 +63,8 @@ void free_ctx(fe_ctx* c) {
                     (void*)c->partial, (void*)c->stats_partial, (void*)c->nbr, (void*)c->nbr_count,
                     (void*)c->sched[0], (void*)c->sched[1], (void*)c->mkeys[0], (void*)c->mkeys[1],
                     (void*)c->inv, (void*)c->pos_m, (void*)c->vb_m, (void*)c->va_m, (void*)c->dens_m,
-                    (void*)c->prho_m, (void*)c->pacc_m})
+                    (void*)c->pa_m, (void*)c->pa2_m, (void*)c->fac_m, (void*)c->src_m, (void*)c->pres_m,
+                    (void*)c->win_meta, (void*)c->win_round, (void*)c->win_delta, (void*)c->nbr16})
         if (p) (void)hipFree(p);
     fe::sort_scratch_free(&c->sort);
     fe::learn_free(c);
@@ -127,7 +128,10 @@ fe_status upload_scene_tab(fe_ctx* c) {
     if (c->nscenes <= 1) return FE_OK;
     std::vector<fe::Sph> tab(c->nscenes);
     for (uint32_t k = 0; k < c->nscenes; k++) {
-        tab[k] = fe::sph_from(c->scene_params[k], c->scene_n[k], c->scene_T[k], c->scene_tab0[k], c->scene_part0[k]);
+        uint32_t fast = 0;
+        FE_HIP(fe::exact_math_flags(c, c->scene_params[k], &fast), "exact arithmetic check");
+        tab[k] = fe::sph_from(c->scene_params[k], c->scene_n[k], c->scene_T[k], c->scene_tab0[k], c->scene_part0[k],
+                              fast);
         tab[k].thr[0] = (unsigned long long)((double)c->options.density_tol * (double)c->scene_n[k] * 4294967296.0);
         tab[k].thr[1] = (unsigned long long)((double)c->options.divergence_tol * (double)c->scene_n[k] * 4294967296.0);
     }
@@ -485,7 +489,13 @@ static fe_status alloc_ctx(fe_ctx* c, const float* xyz) {
         ALLOC(c->mkeys[k], sizeof(uint32_t) * (size_t)n);
     }
     ALLOC(c->inv, s1); ALLOC(c->pos_m, v4); ALLOC(c->vb_m, v4); ALLOC(c->va_m, v4);
-    ALLOC(c->dens_m, s1); ALLOC(c->prho_m, s1); ALLOC(c->pacc_m, v4);
+    ALLOC(c->dens_m, s1); ALLOC(c->pa_m, v4); ALLOC(c->pa2_m, 8 * (size_t)n);
+    ALLOC(c->fac_m, s1); ALLOC(c->src_m, s1); ALLOC(c->pres_m, s1);
+    const size_t nwin = ((size_t)n + fe::kWin - 1) / fe::kWin;
+    ALLOC(c->win_meta, sizeof(uint32_t) * nwin);
+    ALLOC(c->win_round, sizeof(uint4) * fe::kWinRounds * nwin);
+    ALLOC(c->win_delta, sizeof(uint32_t) * fe::kWinMaxRuns * nwin);
+    ALLOC(c->nbr16, sizeof(uint16_t) * fe::kMaxNbr * fe::kWin * nwin);   // whole windows
 #undef ALLOC
     e = fe::sort_scratch_alloc(&c->sort, c->T);
     if (e != hipSuccess) return hip_fail(e, "sort scratch");
@@ -560,6 +570,10 @@ fe_status fe_create(const fe_params* params, const float* xyz, uint32_t n, int d
         c->log2T++;
     }
     fe_status st = alloc_ctx(c, xyz);
+    if (st == FE_OK) {
+        hipError_t ee = fe::exact_math_flags(c, c->params, &c->exact_fast);
+        if (ee != hipSuccess) st = hip_fail(ee, "exact arithmetic check");
+    }
     if (st != FE_OK) {
         free_ctx(c);
         return st;
@@ -582,8 +596,12 @@ fe_status fe_set_params(fe_ctx* c, const fe_params* p) {
     if (!p) return fail(FE_ERR_INVALID_ARGUMENT, "null params");
     if (!(p->smoothing_radius > 0.0f)) return fail(FE_ERR_INVALID_ARGUMENT, "smoothing_radius must be > 0");
     if (c->nscenes > 1) return fail(FE_ERR_UNSUPPORTED, "a batch takes per-scene parameters (fe_batch_set_params)");
+    DeviceGuard g(c->device);
     drop_graph(c);
+    uint32_t fast = 0;
+    FE_HIP(fe::exact_math_flags(c, *p, &fast), "exact arithmetic check");
     c->params = *p;
+    c->exact_fast = fast;
     c->morton_bits = morton_bits_for(*p);
     return FE_OK;
 }
@@ -651,11 +669,13 @@ fe_status fe_upload(fe_ctx* c, int field, const void* src, size_t bytes) {
         return fail(FE_ERR_UNSUPPORTED, "a batch's grid fields are per scene (fe_batch_download)");
     // schedule-ordered mirrors of the uploaded field are stale now
     switch (field) {
-        case FE_FIELD_POSITION_B: c->mirrors &= ~fe::kMirrorPos; break;
+        case FE_FIELD_POSITION_B: c->mirrors &= ~(fe::kMirrorPos | fe::kMirrorPacc); break;
         case FE_FIELD_VELOCITY_B: c->mirrors &= ~fe::kMirrorVelB; break;
         case FE_FIELD_VELOCITY_A: c->mirrors &= ~fe::kMirrorVelA; break;
         case FE_FIELD_DENSITIES: c->mirrors &= ~(fe::kMirrorDens | fe::kMirrorPrho); break;
-        case FE_FIELD_PRESSURES: c->mirrors &= ~fe::kMirrorPrho; break;
+        case FE_FIELD_FACTORS: c->mirrors &= ~fe::kMirrorFac; break;
+        case FE_FIELD_SOURCE_TERMS: c->mirrors &= ~fe::kMirrorSrc; break;
+        case FE_FIELD_PRESSURES: c->mirrors &= ~(fe::kMirrorPrho | fe::kMirrorPres); break;
         case FE_FIELD_PRESSURE_ACCELERATIONS: c->mirrors &= ~fe::kMirrorPacc; break;
         default: break;
     }
@@ -985,6 +1005,7 @@ fe_status fe_batch_create(const fe_params* params, const float* const* xyz, cons
         if (e == hipSuccess) e = hipMalloc((void**)&c->scene_stats, sizeof(uint32_t) * 3 * scenes);
         if (e == hipSuccess) e = hipMemcpy(c->sid, sid.data(), n_all, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemset(c->scene_stats, 0, sizeof(uint32_t) * 3 * scenes);
+        if (e == hipSuccess) e = fe::exact_math_flags(c, c->params, &c->exact_fast);
         if (e != hipSuccess) st = hip_fail(e, "batch tables");
     }
     if (st == FE_OK) st = upload_scene_tab(c);

@@ -22,8 +22,18 @@ struct Sph {
     float gx, gy, gz;
     float mnx, mny, mnz, mxx, mxy, mxz;   // box_min, box_max
     int c0x, c0y, c0z;                     // origin cell of the Morton schedule key
+    uint32_t fast;                         // kFast* bits: exact shortcuts valid for this scene
     unsigned long long thr[2];             // converge thresholds (density, divergence)
 };
+
+// Exact shortcuts of the pair arithmetic, enabled per scene once shown to give the IEEE
+// results for every input the kernels can pass them:
+//  kFastDivH:   r / h as q0 = r (1/h), q0 + (r - q0 h)(1/h) with fused residuals, checked on
+//               the device against the IEEE quotient for every float r in [1e-6, 1.001 h];
+//  kFastDivBox: the gradient division gf / (h r) as the IEEE expansion's reciprocal and
+//               residual steps alone: its operand scaling and final fix-up are identities
+//               when both operands lie in [2^-40, 2^40), which the scene's h and mass bound.
+constexpr uint32_t kFastDivH = 1u, kFastDivBox = 2u;
 
 // The scene table of a batch (tab == nullptr: one scene, whose constants are passed by value).
 struct Scenes {
@@ -37,8 +47,8 @@ __device__ __forceinline__ Sph scene_at(const Scenes& sc, const Sph& s0, uint32_
 }
 __device__ __forceinline__ uint32_t scene_id(const Scenes& sc, uint32_t idx) { return sc.tab ? sc.sid[idx] : 0u; }
 
-// Scene constants from a parameter block (host side).
-inline Sph sph_from(const fe_params& p, uint32_t n, uint32_t T, uint32_t tab0, uint32_t part0) {
+// Scene constants from a parameter block (host side); fast: the scene's kFast* bits.
+inline Sph sph_from(const fe_params& p, uint32_t n, uint32_t T, uint32_t tab0, uint32_t part0, uint32_t fast) {
     Sph s;
     s.h = p.smoothing_radius;
     s.hh = s.h * s.h;
@@ -67,11 +77,12 @@ inline Sph sph_from(const fe_params& p, uint32_t n, uint32_t T, uint32_t tab0, u
     s.c0x = (int)floorf(p.box_min[0] * s.inv_h) - 1;
     s.c0y = (int)floorf(p.box_min[1] * s.inv_h) - 1;
     s.c0z = (int)floorf(p.box_min[2] * s.inv_h) - 1;
+    s.fast = fast;
     s.thr[0] = s.thr[1] = 0ull;
     return s;
 }
 
-inline Sph sph_of(const fe_ctx* c) { return sph_from(c->params, c->n, c->T, 0u, 0u); }
+inline Sph sph_of(const fe_ctx* c) { return sph_from(c->params, c->n, c->T, 0u, 0u, c->exact_fast); }
 
 inline Scenes scenes_of(const fe_ctx* c) { return Scenes{c->scene_tab, c->sid, c->nscenes}; }
 
@@ -122,35 +133,87 @@ __device__ __forceinline__ unsigned long long residual_q(float s_minus_ap, float
     return (unsigned long long)(e * 4294967296.0f);
 }
 
-// Hardware square root (1 ulp) for pair distances.
+// Hardware square root (1 ulp) for pair distances of the approximate kernels.
 __device__ __forceinline__ float pair_sqrt(float r2) { return __builtin_amdgcn_sqrtf(r2); }
 
-// common.glsl:38-54 kernel_w with IEEE q = r / h.
-__device__ __forceinline__ float exact_w(float r, const Sph& s) {
-    float q = r / s.h;
-    if (q >= 1.0f) return 0.0f;
-    if (q <= 0.5f) {
-        float q2 = q * q;
-        float q3 = q2 * q;
-        return s.k * (6.0f * q3 - 6.0f * q2 + 1.0f);
-    }
-    float omq = 1.0f - q;
-    float omq3 = omq * omq * omq;
-    return s.k * 2.0f * omq3;
+// The correctly rounded square root: the hardware result s (within 1 ulp) moved to s - ulp or
+// s + ulp by the signs of the fused residuals x - (s -/+ ulp) s. This is the compiler's IEEE
+// expansion without its rescaling of 0 < x < 2^-96 (such x take sqrtf) and without its
+// pass-through of infinities and NaN, which no pair distance is.
+__device__ __forceinline__ float sqrt_corrected(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rm = fmaf(-sm, s, x);
+    const float rp = fmaf(-sp, s, x);
+    const float t = rm <= 0.0f ? sm : s;
+    return rp > 0.0f ? sp : t;
+}
+__device__ __forceinline__ float sqrt_exact(float x) {
+    if (__builtin_expect(x > 0.0f && x < 0x1p-96f, 0)) return sqrtf(x);
+    return sqrt_corrected(x);
+}
+// The pair distance of a gradient term: sqrt_corrected alone. For x < 2^-96 it may differ
+// from sqrtf(x), but both are below 1e-6, where every gradient term is zero or skipped.
+__device__ __forceinline__ float sqrt_grad(float x) { return sqrt_corrected(x); }
+
+// n / d correctly rounded for 2^-40 <= |n|, |d| < 2^40: the reciprocal refinement and the
+// two residual corrections of the compiler's IEEE expansion, whose operand scaling and
+// fix-up steps change nothing inside that range.
+__device__ __forceinline__ float div_boxed(float n, float d) {
+    float y = __builtin_amdgcn_rcpf(d);
+    const float e = fmaf(-d, y, 1.0f);
+    y = fmaf(e, y, y);
+    float q = n * y;
+    float r = fmaf(-d, q, n);
+    q = fmaf(r, y, q);
+    r = fmaf(-d, q, n);
+    return fmaf(r, y, q);
 }
 
-// common.glsl:56-73 kernel_grad; the scalar that multiplies r_vec.
+// r / h by the residual-corrected product with 1/h (r = 0 or 1e-6 <= r <= 1.001 h).
+__device__ __forceinline__ float div_by_h(float r, const Sph& s) {
+    const float q0 = r * s.inv_h;
+    const float e = fmaf(-q0, s.h, r);
+    return fmaf(e, s.inv_h, q0);
+}
+
+// q = r / h (IEEE) for a pair distance r <= 1.001 h.
+__device__ __forceinline__ float q_of(float r, const Sph& s) {
+    if ((s.fast & kFastDivH) && !(r > 0.0f && r < 1e-6f)) return div_by_h(r, s);
+    return r / s.h;
+}
+
+// common.glsl:38-54 kernel_w with IEEE q = r / h (r <= 1.001 h); both polynomial pieces are
+// evaluated and one is selected.
+__device__ __forceinline__ float exact_w(float r, const Sph& s) {
+    const float q = q_of(r, s);
+    const float q2 = q * q;
+    const float q3 = q2 * q;
+    const float inner = s.k * (6.0f * q3 - 6.0f * q2 + 1.0f);
+    const float omq = 1.0f - q;
+    const float omq3 = omq * omq * omq;
+    const float outer = s.k * 2.0f * omq3;
+    const float w = q <= 0.5f ? inner : outer;
+    return q >= 1.0f ? 0.0f : w;
+}
+
+// common.glsl:56-73 kernel_grad; the scalar that multiplies r_vec (r <= 1.001 h). It is 0 for
+// r < 1e-6, so q needs to be exact only from 1e-6 on: the residual-corrected product serves
+// without a range test.
 __device__ __forceinline__ float exact_grad(float r, const Sph& s) {
-    float q = r / s.h;
-    if (q >= 1.0f || r < 1e-6f) return 0.0f;
-    float gf;
-    if (q <= 0.5f) {
-        gf = s.k * (18.0f * q * q - 12.0f * q);
-    } else {
-        float omq = 1.0f - q;
-        gf = s.k * -6.0f * omq * omq;
-    }
-    return gf / (s.h * r);
+    const float q = (s.fast & kFastDivH) ? div_by_h(r, s) : r / s.h;
+    const float inner = s.k * (18.0f * q * q - 12.0f * q);
+    const float omq = 1.0f - q;
+    const float outer = s.k * -6.0f * omq * omq;
+    const float gf = q <= 0.5f ? inner : outer;
+    const float d = s.h * r;
+    float g;
+    if (s.fast & kFastDivBox)
+        g = div_boxed(gf, d);
+    else
+        g = gf / d;
+    return (q >= 1.0f || r < 1e-6f) ? 0.0f : g;
 }
 
 // p / rho^2 of a particle whose density is <= 1e-6: pressure_force skips such neighbours.

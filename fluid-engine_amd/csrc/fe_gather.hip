@@ -16,8 +16,9 @@
 // so a 128-byte line of a scalar field holds particles of four unrelated cells. The work
 // is therefore scheduled in Morton order of the particles' cells (work-item t handles
 // particle sched[t]) and the data a gather reads from its neighbours is kept in
-// schedule-ordered mirrors (pos_m, vb_m, va_m, dens_m, prho_m = p / rho^2, pacc_m) that the
-// producing stage writes next to its hash-ordered output. The list holds schedule
+// schedule-ordered mirrors that the producing stage writes next to its hash-ordered output:
+// pos_m (position_b, with p / rho^2 in w), vb_m, va_m, dens_m, and pa_m / pa2_m (position
+// and pressure acceleration packed for pressure_update). The list holds schedule
 // positions in tiles of 16 work-items x 4 slots, so 16 lanes reading four entries each
 // read 256 contiguous bytes, a cell group's entries fill whole lines while the group is
 // built, and each lane keeps four independent neighbour loads in flight.
@@ -74,18 +75,22 @@ __device__ __forceinline__ void visit27(const uint2* __restrict__ range, const f
 
 // ---- mirrors -----------------------------------------------------------------------------
 
-// After the schedule sort: inverse schedule and the position / velocity_b mirrors.
+// After the schedule sort: inverse schedule and the position, velocity_b and velocity_a
+// mirrors (velocity_a is not permuted by a sort, so its mirror holds velocity_a at the new
+// order's indices, which is what divergence_source reads).
 __global__ __launch_bounds__(kBlock) void k_schedule_mirrors(const uint32_t* __restrict__ sched, uint32_t n,
                                                              const float4* __restrict__ pos_b,
                                                              const float4* __restrict__ vel_b,
+                                                             const float4* __restrict__ vel_a,
                                                              uint32_t* __restrict__ inv, float4* __restrict__ pos_m,
-                                                             float4* __restrict__ vb_m) {
+                                                             float4* __restrict__ vb_m, float4* __restrict__ va_m) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = sched[t];
     inv[i] = t;
     pos_m[t] = pos_b[i];
     vb_m[t] = vel_b[i];
+    va_m[t] = vel_a[i];
 }
 
 // reorder.comp:44 for the velocities alone, written to velocity_b and its mirror.
@@ -108,21 +113,29 @@ __global__ __launch_bounds__(kBlock) void k_refresh_mirrors(const uint32_t* __re
                                                             const float4* __restrict__ vel_a,
                                                             const float* __restrict__ dens,
                                                             const float* __restrict__ pres,
-                                                            const float4* __restrict__ pacc, uint32_t* __restrict__ inv,
-                                                            float4* __restrict__ pos_m, float4* __restrict__ vb_m,
-                                                            float4* __restrict__ va_m, float* __restrict__ dens_m,
-                                                            float* __restrict__ prho_m, float4* __restrict__ pacc_m) {
+                                                            const float4* __restrict__ pacc,
+                                                            const float* __restrict__ fac, const float* __restrict__ src,
+                                                            uint32_t* __restrict__ inv, float4* __restrict__ pos_m,
+                                                            float4* __restrict__ vb_m, float4* __restrict__ va_m,
+                                                            float* __restrict__ dens_m, float4* __restrict__ pa_m,
+                                                            float2* __restrict__ pa2_m, float* __restrict__ fac_m,
+                                                            float* __restrict__ src_m, float* __restrict__ pres_m) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = sched[t];
     inv[i] = t;
-    pos_m[t] = pos_b[i];
+    const float4 x = pos_b[i];
+    const float rho = dens[i], p = pres[i];
+    pos_m[t] = make_float4(x.x, x.y, x.z, prho_of(p, rho));
     vb_m[t] = vel_b[i];
     va_m[t] = vel_a[i];
-    const float rho = dens[i];
     dens_m[t] = rho;
-    prho_m[t] = prho_of(pres[i], rho);
-    pacc_m[t] = pacc[i];
+    const float4 a = pacc[i];
+    pa_m[t] = make_float4(x.x, x.y, x.z, a.x);
+    pa2_m[t] = make_float2(a.y, a.z);
+    fac_m[t] = fac[i];
+    src_m[t] = src[i];
+    pres_m[t] = p;
 }
 
 // ---- neighbour list build -----------------------------------------------------------------
@@ -283,10 +296,216 @@ __global__ __launch_bounds__(64 * kWaves) void k_build_list(Sph s0, Scenes sc, u
     }
 }
 
+// ---- sweep windows ----------------------------------------------------------------------
+// A Jacobi sweep reads, for every listed pair, the neighbour's position and one solver
+// field. The 64 particles of a window (64 consecutive schedule positions) list ~2100 pairs
+// but only ~300 distinct neighbours, which lie in a few dozen runs of consecutive schedule
+// positions (a cell's particles are consecutive in the Morton schedule). k_window_tables
+// finds that union once per list build: it inserts the window's listed schedule positions
+// into an LDS hash set, takes the runs of present positions (a gap of at most G absent
+// positions inside a run is staged too), lays the runs out one after another as the
+// window's staging slots, and rewrites each list entry as a 16-bit staging slot. A sweep
+// then stages the window's slots with coalesced loads into LDS and walks the 16-bit lists
+// over LDS. The staged values are the mirrors' own values and each particle still visits
+// its pairs in list order, so the sums are unchanged bit for bit. A window whose union
+// needs more than kWinMaxK slots or kWinMaxRuns runs, or that holds a particle with more
+// than kMaxNbr pairs, is marked kWinFallback and swept from the 32-bit list.
+
+constexpr uint32_t kHashSlots = 1024;   // > 2 kWinMaxK, so probes stay short
+constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
+
+struct WindowStage {
+    uint32_t key[kHashSlots];    // schedule position, or kHashEmpty
+    uint32_t val[kHashSlots];    // run << 16 | position - run start
+    uint32_t rstart[kWinMaxRuns];
+    uint32_t rlen[kWinMaxRuns];
+    uint32_t roff[kWinMaxRuns];
+    unsigned long long mask[kWinRounds];
+    uint32_t rbase[kWinRounds];
+    uint32_t count, over;
+};
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t x) { return (x * 2654435761u) >> 22; }
+
+__device__ __forceinline__ uint32_t hash_find(const WindowStage& st, uint32_t x) {
+    uint32_t h = hash_slot(x);
+    for (uint32_t p = 0; p < kHashSlots; p++) {
+        const uint32_t k = st.key[h];
+        if (k == x) return h;
+        if (k == kHashEmpty) return kHashEmpty;
+        h = (h + 1u) & (kHashSlots - 1u);
+    }
+    return kHashEmpty;
+}
+
+__device__ __forceinline__ void hash_insert(WindowStage& st, uint32_t x) {
+    uint32_t h = hash_slot(x);
+    for (uint32_t p = 0; p < kHashSlots; p++) {
+        const uint32_t old = atomicCAS(&st.key[h], kHashEmpty, x);
+        if (old == kHashEmpty) {
+            atomicAdd(&st.count, 1u);
+            return;
+        }
+        if (old == x) return;
+        h = (h + 1u) & (kHashSlots - 1u);
+    }
+    st.over = 1u;
+}
+
+// Runs of the hash set with gaps of at most G: run starts in slot order, their lengths and
+// staging offsets. Returns the number of staging slots K, or kWinFallback when the runs do
+// not fit.
+__device__ uint32_t window_runs(WindowStage& st, uint32_t G, uint32_t& nruns_out) {
+    const int lane = threadIdx.x & 63;
+    uint32_t nruns = 0;
+    for (uint32_t s0 = 0; s0 < kHashSlots; s0 += 64) {
+        const uint32_t x = st.key[s0 + lane];
+        bool start = x != kHashEmpty;
+        for (uint32_t d = 1; start && d <= G + 1u; d++)
+            if (x >= d && hash_find(st, x - d) != kHashEmpty) start = false;
+        const uint64_t b = __ballot(start);
+        if (start) {
+            const uint32_t rid = nruns + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+            if (rid < kWinMaxRuns) st.rstart[rid] = x;
+        }
+        nruns += (uint32_t)__popcll(b);
+    }
+    nruns_out = nruns;
+    if (nruns > kWinMaxRuns) return kWinFallback;
+    wave_sync();
+    for (uint32_t r = lane; r < nruns; r += 64) {
+        const uint32_t x0 = st.rstart[r];
+        st.val[hash_find(st, x0)] = r << 16;
+        uint32_t last = x0;
+        for (;;) {
+            uint32_t y = last + 1u, h = kHashEmpty;
+            for (; y <= last + G + 1u; y++)
+                if ((h = hash_find(st, y)) != kHashEmpty) break;
+            if (h == kHashEmpty) break;
+            if (y - x0 >= kWinMaxK) {   // longer than the staging area: K exceeds kWinMaxK
+                last = x0 + kWinMaxK;
+                break;
+            }
+            st.val[h] = (r << 16) | (y - x0);
+            last = y;
+        }
+        st.rlen[r] = last - x0 + 1u;
+    }
+    wave_sync();
+    uint32_t carry = 0;
+    for (uint32_t r0 = 0; r0 < nruns; r0 += 64) {
+        const uint32_t r = r0 + lane;
+        const uint32_t len = r < nruns ? st.rlen[r] : 0u;
+        const uint32_t ex = wave_excl_scan(len);
+        if (r < nruns) st.roff[r] = carry + ex;
+        carry += __shfl(ex + len, 63, 64);
+    }
+    wave_sync();
+    return carry <= kWinMaxK ? carry : kWinFallback;
+}
+
+// The 16-bit list of work-item t: tiles of 16 work-items x 8 slots (uint4 chunk k / 8).
+__device__ __forceinline__ uint4* list16_tile(uint16_t* nbr16, uint32_t t) {
+    return reinterpret_cast<uint4*>(nbr16) + (size_t)(t >> 4) * (kMaxNbr / 8) * 16 + (t & 15u);
+}
+__device__ __forceinline__ const uint4* list16_tile(const uint16_t* nbr16, uint32_t t) {
+    return reinterpret_cast<const uint4*>(nbr16) + (size_t)(t >> 4) * (kMaxNbr / 8) * 16 + (t & 15u);
+}
+
+template <int kWaves>
+__global__ __launch_bounds__(64 * kWaves) void k_window_tables(uint32_t n, const uint32_t* __restrict__ nbr,
+                                                               const uint32_t* __restrict__ cnt,
+                                                               uint32_t* __restrict__ win_meta,
+                                                               uint4* __restrict__ win_round,
+                                                               uint32_t* __restrict__ win_delta,
+                                                               uint16_t* __restrict__ nbr16) {
+    __shared__ WindowStage stage[kWaves];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t win = xcd_block() * kWaves + wv;
+    if (win * kWin >= n) return;
+    WindowStage& st = stage[wv];
+    const uint32_t t = win * kWin + lane;
+    const bool valid = t < n;
+    const uint32_t m = valid ? cnt[t] : 0u;
+    if (__any(m > kMaxNbr)) {
+        if (lane == 0) win_meta[win] = kWinFallback;
+        return;
+    }
+    for (uint32_t q = lane; q < kHashSlots; q += 64) st.key[q] = kHashEmpty;
+    if (lane == 0) {
+        st.count = 0;
+        st.over = 0;
+    }
+    wave_sync();
+    const uint4* tile = list_tile(nbr, t);
+    for (uint32_t k = 0; k < m; k += 4) {
+        if (st.count > kWinMaxK) break;   // the window falls back in any case
+        const uint4 q4 = tile[(k >> 2) * 16];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++)
+            if (k + u < m) hash_insert(st, list_pick(q4, u));
+    }
+    wave_sync();
+    uint32_t K = kWinFallback, nruns = 0;
+    if (st.count <= kWinMaxK && !st.over) {
+        // gap 4 stages ~30% more slots in half as many runs; narrower gaps when that overflows
+        for (uint32_t G : {4u, 1u, 0u}) {
+            K = window_runs(st, G, nruns);
+            if (K != kWinFallback) break;
+        }
+    }
+    if (K == kWinFallback) {
+        if (lane == 0) win_meta[win] = kWinFallback;
+        return;
+    }
+    // staging rounds: the runs that start inside each round, and the run holding its first slot
+    if (lane < (int)kWinRounds) st.mask[lane] = 0ull;
+    wave_sync();
+    for (uint32_t r = lane; r < nruns; r += 64) {
+        const uint32_t off = st.roff[r], len = st.rlen[r];
+        if (off & 63u) atomicOr(&st.mask[off >> 6], 1ull << (off & 63u));
+        for (uint32_t k = (off + 63u) >> 6; k * 64u < off + len; k++) st.rbase[k] = r;
+    }
+    wave_sync();
+    const uint32_t rounds = (K + 63u) >> 6;
+    if (lane < (int)rounds) {
+        const unsigned long long mk = st.mask[lane];
+        win_round[(size_t)win * kWinRounds + lane] =
+            make_uint4((uint32_t)mk, (uint32_t)(mk >> 32), st.rbase[lane], 0u);
+    }
+    for (uint32_t r = lane; r < nruns; r += 64) win_delta[(size_t)win * kWinMaxRuns + r] = st.rstart[r] - st.roff[r];
+    if (lane == 0) win_meta[win] = K | (nruns << 16);
+    // the list as staging slots (slots past the pair count hold 0)
+    if (valid) {
+        uint4* out = list16_tile(nbr16, t);
+        for (uint32_t k = 0; k < m; k += 8) {
+            uint32_t loc[8];
+            const uint4 qa = tile[(k >> 2) * 16];
+            const uint4 qb = k + 4 < m ? tile[((k >> 2) + 1) * 16] : qa;
+#pragma unroll
+            for (uint32_t u = 0; u < 8; u++) {
+                loc[u] = 0u;
+                if (k + u < m) {
+                    const uint32_t x = list_pick(u < 4 ? qa : qb, u & 3u);
+                    const uint32_t v = st.val[hash_find(st, x)];
+                    loc[u] = st.roff[v >> 16] + (v & 0xFFFFu);
+                }
+            }
+            out[(k >> 3) * 16] = make_uint4(loc[0] | (loc[1] << 16), loc[2] | (loc[3] << 16),
+                                            loc[4] | (loc[5] << 16), loc[6] | (loc[7] << 16));
+        }
+    }
+}
+
 // ---- list consumers: one policy per reference shader ------------------------------------
 // pair() repeats the reference's own tests (r^2 > h^2 rejects, gradient terms need
 // r > 1e-6), so the same code serves list entries (which pass them) and the bucket walk.
-// load(jt) reads neighbour data by schedule position jt.
+// The tests select instead of branching: a skipped pair adds +0 or -0. Every sum starts at
+// +0 and in round-to-nearest only -0 + -0 gives -0, so a sum is never -0 and adding a zero
+// leaves it unchanged bit for bit.
+// geo_of() gives the record that holds a neighbour's position (pos_m, or pa_m for
+// pressure_update); load(jt, g) reads the rest of the neighbour's data by schedule position
+// jt, given that record g, and self(i, t, g) the particle's own.
 
 // density_and_alpha.comp:14-72 over every pair with r^2 <= h^2 (the list includes the
 // particle itself and any r <= 1e-6 pair, which add to the density only).
@@ -294,32 +513,36 @@ struct DensityAlpha {
     float* dens;
     float* fac;
     float* dens_m;
+    float* fac_m;
     struct Self {};
     struct Cand {};
     struct Acc { float rho, sgs, gx, gy, gz; };
-    __device__ Self self(uint32_t, uint32_t) const { return {}; }
-    __device__ Cand load(uint32_t) const { return {}; }
+    static constexpr bool kPaGeo = false;
+    __device__ Self self(uint32_t, uint32_t, const float4&) const { return {}; }
+    __device__ Cand load(uint32_t, const float4&) const { return {}; }
     __device__ Acc zero() const { return {0.f, 0.f, 0.f, 0.f, 0.f}; }
-    __device__ void pair(Acc& a, const Self&, const Sph& s, float rx, float ry, float rz, const Cand&) const {
+    __device__ __forceinline__ void pair(Acc& a, const Self&, const Sph& s, float rx, float ry, float rz,
+                                         const Cand&) const {
         const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 > s.hh) return;
-        const float r = sqrtf(r2);
+        const bool in = !(r2 > s.hh);
+        const float r = sqrt_exact(r2);
         const float w = exact_w(r, s);
-        a.rho += s.mass * w;
-        if (r > 1e-6f) {
-            const float g = exact_grad(r, s);
-            const float mx = s.mass * (g * rx), my = s.mass * (g * ry), mz = s.mass * (g * rz);
-            a.gx += mx;
-            a.gy += my;
-            a.gz += mz;
-            a.sgs += mx * mx + my * my + mz * mz;
-        }
+        a.rho += in ? s.mass * w : 0.0f;
+        const bool gok = in && r > 1e-6f;
+        const float g = exact_grad(r, s);
+        const float mx = s.mass * (g * rx), my = s.mass * (g * ry), mz = s.mass * (g * rz);
+        a.gx += gok ? mx : 0.0f;
+        a.gy += gok ? my : 0.0f;
+        a.gz += gok ? mz : 0.0f;
+        a.sgs += gok ? mx * mx + my * my + mz * mz : 0.0f;
     }
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         dens[i] = a.rho;
         dens_m[t] = a.rho;
         const float alpha = a.sgs + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
-        fac[i] = alpha > 1e-6f ? alpha : 0.0f;
+        const float f = alpha > 1e-6f ? alpha : 0.0f;
+        fac[i] = f;
+        fac_m[t] = f;
         return 0;
     }
 };
@@ -333,20 +556,20 @@ struct Viscosity {
     struct Self { float4 v; };
     struct Cand { float4 v; float rho; };
     struct Acc { float x, y, z; };
-    __device__ Self self(uint32_t, uint32_t t) const { return {vb_m[t]}; }
-    __device__ Cand load(uint32_t jt) const { return {vb_m[jt], dens_m[jt]}; }
+    static constexpr bool kPaGeo = false;
+    __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {vb_m[t]}; }
+    __device__ Cand load(uint32_t jt, const float4&) const { return {vb_m[jt], dens_m[jt]}; }
     __device__ Acc zero() const { return {0.f, 0.f, 0.f}; }
-    __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
+    __device__ __forceinline__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
+                                         const Cand& c) const {
         const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 > s.hh) return;
-        const float r = sqrtf(r2);
+        const float r = sqrt_exact(r2);
+        const bool ok = !(r2 > s.hh) && r > 1e-6f;
         const float w = exact_w(r, s);
-        if (r > 1e-6f) {
-            const float m_rho = s.mass / c.rho;
-            a.x += (m_rho * (c.v.x - si.v.x)) * w;
-            a.y += (m_rho * (c.v.y - si.v.y)) * w;
-            a.z += (m_rho * (c.v.z - si.v.z)) * w;
-        }
+        const float m_rho = s.mass / c.rho;
+        a.x += ok ? (m_rho * (c.v.x - si.v.x)) * w : 0.0f;
+        a.y += ok ? (m_rho * (c.v.y - si.v.y)) * w : 0.0f;
+        a.z += ok ? (m_rho * (c.v.z - si.v.z)) * w : 0.0f;
     }
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si, const Sph& s) const {
         const float vx = si.v.x + s.nu * a.x, vy = si.v.y + s.nu * a.y, vz = si.v.z + s.nu * a.z;
@@ -363,22 +586,24 @@ struct DensitySource {
     const float* dens;
     float* src;
     float* pres;
-    float* prho_m;
+    float4* pos_w;   // the position mirror, whose w takes p / rho^2
+    float* src_m;
+    float* pres_m;
     struct Self { float4 v; };
     struct Cand { float4 v; };
     struct Acc { float div; };
-    __device__ Self self(uint32_t, uint32_t t) const { return {va_m[t]}; }
-    __device__ Cand load(uint32_t jt) const { return {va_m[jt]}; }
+    static constexpr bool kPaGeo = false;
+    __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {va_m[t]}; }
+    __device__ Cand load(uint32_t jt, const float4&) const { return {va_m[jt]}; }
     __device__ Acc zero() const { return {0.f}; }
-    __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
+    __device__ __forceinline__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
+                                         const Cand& c) const {
         const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 > s.hh) return;
-        const float r = sqrtf(r2);
-        if (r > 1e-6f) {
-            const float g = exact_grad(r, s);
-            const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
-            a.div += s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz));
-        }
+        const float r = sqrt_grad(r2);
+        const bool ok = !(r2 > s.hh) && r > 1e-6f;
+        const float g = exact_grad(r, s);
+        const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
+        a.div += ok ? s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz)) : 0.0f;
     }
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         const float rho = dens[i];
@@ -386,41 +611,48 @@ struct DensitySource {
         if (s.dt > 1e-6f) source = ((s.rho0 - rho) / s.dt) - a.div;
         src[i] = source;
         pres[i] = 0.0f;
-        prho_m[t] = prho_of(0.0f, rho);
+        src_m[t] = source;
+        pres_m[t] = 0.0f;
+        pos_w[t].w = prho_of(0.0f, rho);
         return 0;
     }
 };
 
 // divergence_source_term.comp:14-62. velocity_a still holds the previous sort order and is
-// read at the current order's indices (as the reference does), so it is gathered through
-// the hash index sched[jt]. Also seeds p / rho^2 for the divergence solve from the
-// pressures the density solve left (never reset, never permuted).
+// read at the current order's indices (as the reference does): its mirror va_m, written
+// after the sort as velocity_a[sched[t]], holds exactly those values. Also seeds p / rho^2
+// for the divergence solve from the pressures the density solve left (never reset, never
+// permuted).
 struct DivergenceSource {
-    const float4* vel_a;
-    const uint32_t* sched;
+    const float4* va_m;
     const float* dens;
     const float* pres;
     float* src;
-    float* prho_m;
+    float4* pos_w;
+    float* src_m;
+    float* pres_m;
     struct Self { float4 v; };
     struct Cand { float4 v; };
     struct Acc { float div; };
-    __device__ Self self(uint32_t i, uint32_t) const { return {vel_a[i]}; }
-    __device__ Cand load(uint32_t jt) const { return {vel_a[sched[jt]]}; }
+    static constexpr bool kPaGeo = false;
+    __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {va_m[t]}; }
+    __device__ Cand load(uint32_t jt, const float4&) const { return {va_m[jt]}; }
     __device__ Acc zero() const { return {0.f}; }
-    __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
+    __device__ __forceinline__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
+                                         const Cand& c) const {
         const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 > s.hh) return;
-        const float r = sqrtf(r2);
-        if (r > 1e-6f) {
-            const float g = exact_grad(r, s);
-            const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
-            a.div += s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz));
-        }
+        const float r = sqrt_grad(r2);
+        const bool ok = !(r2 > s.hh) && r > 1e-6f;
+        const float g = exact_grad(r, s);
+        const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
+        a.div += ok ? s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz)) : 0.0f;
     }
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         src[i] = -a.div;
-        prho_m[t] = prho_of(pres[i], dens[i]);
+        src_m[t] = -a.div;
+        const float p = pres[i];
+        pres_m[t] = p;
+        pos_w[t].w = prho_of(p, dens[i]);
         return 0;
     }
 };
@@ -484,13 +716,14 @@ struct DensitySourceFeat {
     struct Self { float4 v; float pp; };
     struct Cand { float4 v; float pp; };
     struct Acc { float div; FeatAcc f; };
-    __device__ Self self(uint32_t, uint32_t t) const { return {base.va_m[t], pprev_m[t]}; }
-    __device__ Cand load(uint32_t jt) const { return {base.va_m[jt], pprev_m[jt]}; }
+    static constexpr bool kPaGeo = false;
+    __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {base.va_m[t], pprev_m[t]}; }
+    __device__ Cand load(uint32_t jt, const float4&) const { return {base.va_m[jt], pprev_m[jt]}; }
     __device__ Acc zero() const { return {0.f, feat_zero()}; }
     __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
         const float r2 = rx * rx + ry * ry + rz * rz;
         if (r2 > s.hh) return;
-        const float r = sqrtf(r2);
+        const float r = sqrt_exact(r2);
         const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
         if (r > 1e-6f) {
             const float g = exact_grad(r, s);
@@ -504,7 +737,9 @@ struct DensitySourceFeat {
         if (s.dt > 1e-6f) source = ((s.rho0 - rho) / s.dt) - a.div;
         base.src[i] = source;
         base.pres[i] = 0.0f;
-        base.prho_m[t] = prho_of(0.0f, rho);
+        base.src_m[t] = source;
+        base.pres_m[t] = 0.0f;
+        base.pos_w[t].w = prho_of(0.0f, rho);
         feat_write(feat, n_, t, s, a.f, rho, fac[i], source, si.pp, s.rho0, s.dt);
         return 0;
     }
@@ -521,13 +756,14 @@ struct DivergenceSourceFeat {
     struct Self { float4 v; float pp; };
     struct Cand { float4 v; float pp; };
     struct Acc { float div; FeatAcc f; };
-    __device__ Self self(uint32_t i, uint32_t t) const { return {base.vel_a[i], pprev_m[t]}; }
-    __device__ Cand load(uint32_t jt) const { return {base.vel_a[base.sched[jt]], pprev_m[jt]}; }
+    static constexpr bool kPaGeo = false;
+    __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {base.va_m[t], pprev_m[t]}; }
+    __device__ Cand load(uint32_t jt, const float4&) const { return {base.va_m[jt], pprev_m[jt]}; }
     __device__ Acc zero() const { return {0.f, feat_zero()}; }
     __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
         const float r2 = rx * rx + ry * ry + rz * rz;
         if (r2 > s.hh) return;
-        const float r = sqrtf(r2);
+        const float r = sqrt_exact(r2);
         const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
         if (r > 1e-6f) {
             const float g = exact_grad(r, s);
@@ -538,81 +774,106 @@ struct DivergenceSourceFeat {
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si, const Sph& s) const {
         const float source = -a.div;
         base.src[i] = source;
-        const float rho = base.dens[i];
-        base.prho_m[t] = prho_of(base.pres[i], rho);
+        base.src_m[t] = source;
+        const float rho = base.dens[i], p = base.pres[i];
+        base.pres_m[t] = p;
+        base.pos_w[t].w = prho_of(p, rho);
         feat_write(feat, n_, t, s, a.f, rho, fac[i], source, si.pp, s.rho0, s.dt);
         return 0;
     }
     uint32_t n_;
 };
 
-// pressure_force.comp:15-78. Neighbours supply p_j / rho_j^2 (kNoRho when rho_j <= 1e-6,
-// which the reference skips).
+// pressure_force.comp:15-78. Neighbours supply p_j / rho_j^2 in the w of their position
+// record (kNoRho when rho_j <= 1e-6, which the reference skips); the particle's own p_i /
+// rho_i^2 is the same field, with kNoRho read as the reference's 0. Writes the pressure
+// acceleration to the hash-ordered field and packed with the position for pressure_update.
+// A staged window entry is the position record itself.
 struct PressureForce {
-    const float* pres;
-    const float* dens;
-    const float* prho_m;
     float4* pacc;
-    float4* pacc_m;
-    struct Self { float prho; };
+    float4* pa_m;
+    float2* pa2_m;
+    static constexpr int kParts = 1;
+    static constexpr bool kPaGeo = false;
+    struct Self { float prho, px, py, pz; };
     struct Cand { float prho; };
     struct Acc { float x, y, z; };
-    __device__ Self self(uint32_t i, uint32_t) const {
-        const float rho = dens[i];
-        float pr = 0.0f;
-        if (rho > 1e-6f) pr = pres[i] / (rho * rho);
-        return {pr};
+    __device__ __forceinline__ Self self(uint32_t, uint32_t, const float4& g) const {
+        return {__float_as_uint(g.w) == kNoRho ? 0.0f : g.w, g.x, g.y, g.z};
     }
-    __device__ Cand load(uint32_t jt) const { return {prho_m[jt]}; }
-    __device__ Acc zero() const { return {0.f, 0.f, 0.f}; }
-    __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
+    __device__ __forceinline__ Cand load(uint32_t, const float4& g) const { return {g.w}; }
+    __device__ __forceinline__ void stage(float4* st, float2*, uint32_t q, uint32_t jt,
+                                          const float4* pos_m) const {
+        st[q] = pos_m[jt];
+    }
+    __device__ __forceinline__ Cand staged(const float2*, uint32_t, const float4& e0) const { return {e0.w}; }
+    __device__ __forceinline__ Acc zero() const { return {0.f, 0.f, 0.f}; }
+    __device__ __forceinline__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
         const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 > s.hh) return;
-        const float r = sqrtf(r2);
-        if (r > 1e-6f && __float_as_uint(c.prho) != kNoRho) {
-            const float g = exact_grad(r, s);
-            const float mt = s.mass * (si.prho + c.prho);
-            a.x += mt * (g * rx);
-            a.y += mt * (g * ry);
-            a.z += mt * (g * rz);
-        }
+        const float r = sqrt_grad(r2);
+        const bool ok = !(r2 > s.hh) && r > 1e-6f && __float_as_uint(c.prho) != kNoRho;
+        const float g = exact_grad(r, s);   // finite, so a skipped pair adds (+0) g r = +/-0
+        const float mt = ok ? s.mass * (si.prho + c.prho) : 0.0f;
+        a.x += mt * (g * rx);
+        a.y += mt * (g * ry);
+        a.z += mt * (g * rz);
     }
-    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
+    __device__ __forceinline__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si, const Sph& s) const {
         const float4 v = make_float4(-a.x, -a.y, -a.z, 0.0f);
         pacc[i] = v;
-        pacc_m[t] = v;
+        pa_m[t] = make_float4(si.px, si.py, si.pz, v.x);
+        pa2_m[t] = make_float2(v.y, v.z);
         return 0;
     }
 };
 
 // pressure_update.comp:18-84 (in place; neighbours' pressures are not read), plus the
-// sweep's quantised compression residual.
+// sweep's quantised compression residual. Positions and pressure accelerations come packed
+// in pa_m / pa2_m; the particle's own factor, source term, density and pressure from their
+// schedule-ordered mirrors. The new pressure goes to the hash-ordered field and its mirror,
+// p / rho^2 to the w of the position mirror. A staged window entry is pa_m and pa2_m.
 struct PressureUpdate {
-    const float4* pacc_m;
-    const float* fac;
-    const float* src;
-    const float* dens;
+    const float4* pa_m;
+    const float2* pa2_m;
+    const float* fac_m;
+    const float* src_m;
+    const float* dens_m;
     float* pres;
-    float* prho_m;
-    struct Self { float4 a; };
-    struct Cand { float4 a; };
+    float* pres_m;
+    float4* pos_w;
+    static constexpr int kParts = 2;
+    static constexpr bool kPaGeo = true;
+    struct Self { float ax, ay, az; };
+    struct Cand { float ax, ay, az; };
     struct Acc { float ap; };
-    __device__ Self self(uint32_t, uint32_t t) const { return {pacc_m[t]}; }
-    __device__ Cand load(uint32_t jt) const { return {pacc_m[jt]}; }
-    __device__ Acc zero() const { return {0.f}; }
-    __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 > s.hh) return;
-        const float r = sqrtf(r2);
-        if (r > 1e-6f) {
-            const float g = exact_grad(r, s);
-            const float dx = si.a.x - c.a.x, dy = si.a.y - c.a.y, dz = si.a.z - c.a.z;
-            a.ap += s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz));
-        }
+    __device__ __forceinline__ Self self(uint32_t, uint32_t t, const float4& g) const {
+        const float2 b = pa2_m[t];
+        return {g.w, b.x, b.y};
     }
-    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
-        const float alpha_i = fac[i], rho_i = dens[i], src_i = src[i];
-        float p = pres[i];
+    __device__ __forceinline__ Cand load(uint32_t jt, const float4& g) const {
+        const float2 b = pa2_m[jt];
+        return {g.w, b.x, b.y};
+    }
+    __device__ __forceinline__ void stage(float4* st, float2* st2, uint32_t q, uint32_t jt, const float4*) const {
+        st[q] = pa_m[jt];
+        st2[q] = pa2_m[jt];
+    }
+    __device__ __forceinline__ Cand staged(const float2* st2, uint32_t loc, const float4& e0) const {
+        const float2 b = st2[loc];
+        return {e0.w, b.x, b.y};
+    }
+    __device__ __forceinline__ Acc zero() const { return {0.f}; }
+    __device__ __forceinline__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float r = sqrt_grad(r2);
+        const bool ok = !(r2 > s.hh) && r > 1e-6f;
+        const float g = exact_grad(r, s);
+        const float dx = si.ax - c.ax, dy = si.ay - c.ay, dz = si.az - c.az;
+        a.ap += ok ? s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz)) : 0.0f;
+    }
+    __device__ __forceinline__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
+        const float alpha_i = fac_m[t], rho_i = dens_m[t], src_i = src_m[t];
+        float p = pres_m[t];
         const float ap = s.dt * a.ap;
         const float err = src_i - ap;
         const unsigned long long q = residual_q(err, s.dt, s.rho0);
@@ -623,12 +884,22 @@ struct PressureUpdate {
                 const float np = p + corr;
                 p = np > 0.0f ? np : 0.0f;
                 pres[i] = p;
+                pres_m[t] = p;
             }
         }
-        prho_m[t] = prho_of(p, rho_i);
+        pos_w[t].w = prho_of(p, rho_i);
         return q;
     }
 };
+
+// The record that holds a neighbour's position: pa_m for pressure_update, else pos_m.
+template <class P>
+__device__ __forceinline__ float4 geo_of(const P& pol, const float4* pos_m, uint32_t jt) {
+    if constexpr (P::kPaGeo)
+        return pol.pa_m[jt];
+    else
+        return pos_m[jt];
+}
 
 template <int B>
 __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v) {
@@ -644,6 +915,30 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v
     return t;
 }
 
+// A sweep block's residual output: the block's sum into partial[blockIdx.x] (one scene), or
+// per-scene sums by atomics into the scenes' slots (integer, so order-independent; one per
+// wave unless the wave straddles two scenes).
+template <bool kBatch, int B>
+__device__ __forceinline__ void sweep_epilogue(unsigned long long q, uint32_t scene,
+                                               unsigned long long* __restrict__ partial,
+                                               unsigned long long* __restrict__ slot_out) {
+    if (partial) {
+        const unsigned long long b = block_sum_u64<B>(q);
+        if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    }
+    if (kBatch && slot_out) {
+        const uint32_t s_first = __shfl(scene, 0, 64);
+        if (__all(scene == s_first)) {
+            unsigned long long v = q;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if ((threadIdx.x & 63) == 0 && v) atomicAdd(slot_out + (size_t)s_first * kMaxSolveIters, v);
+        } else if (q) {
+            atomicAdd(slot_out + (size_t)scene * kMaxSolveIters, q);
+        }
+    }
+}
+
 // One work-item per particle: the listed pairs, four neighbour loads in flight, or the
 // bucket walk when the list is stale or the particle overflowed it. With `prev` set, a
 // converged previous sweep skips the launch; with `partial` set, each block stores the sum
@@ -653,14 +948,14 @@ template <class P>
 __device__ __forceinline__ unsigned long long gather_one(const P& pol, const Sph& s, uint32_t t,
                                                          const uint2* __restrict__ range,
                                                          const float4* __restrict__ pos_b,
-                                                         const float4* __restrict__ pos_m,
+                                                         const float4* pos_m,
                                                          const uint32_t* __restrict__ sched,
                                                          const uint32_t* __restrict__ inv,
                                                          const uint32_t* __restrict__ nbr,
                                                          const uint32_t* __restrict__ cnt, uint32_t list_ok) {
     const uint32_t i = sched[t];
-    const float4 pi = pos_m[t];
-    const typename P::Self si = pol.self(i, t);
+    const float4 pi = geo_of(pol, pos_m, t);
+    const typename P::Self si = pol.self(i, t, pi);
     typename P::Acc acc = pol.zero();
     const uint32_t m = list_ok ? cnt[t] : 0xFFFFFFFFu;
     if (m <= kMaxNbr) {
@@ -672,8 +967,8 @@ __device__ __forceinline__ unsigned long long gather_one(const P& pol, const Sph
             typename P::Cand cj[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                pj[u] = pos_m[jt[u]];
-                cj[u] = pol.load(jt[u]);
+                pj[u] = geo_of(pol, pos_m, jt[u]);
+                cj[u] = pol.load(jt[u], pj[u]);
             }
 #pragma unroll
             for (int u = 0; u < 4; u++)
@@ -681,7 +976,8 @@ __device__ __forceinline__ unsigned long long gather_one(const P& pol, const Sph
         }
     } else {
         visit27(range, pos_b, s, pi.x, pi.y, pi.z, [&](uint32_t j, const float4& pj) {
-            pol.pair(acc, si, s, pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, pol.load(inv[j]));
+            const uint32_t jt = inv[j];
+            pol.pair(acc, si, s, pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, pol.load(jt, geo_of(pol, pos_m, jt)));
         });
     }
     return pol.finish(i, t, acc, si, s);
@@ -695,7 +991,7 @@ template <class P, bool kBatch, int B>
 __global__ __launch_bounds__(B) void k_list_gather(P pol, Sph s0, Scenes sc, uint32_t n,
                                                         const uint2* __restrict__ range,
                                                         const float4* __restrict__ pos_b,
-                                                        const float4* __restrict__ pos_m,
+                                                        const float4* pos_m,
                                                         const uint32_t* __restrict__ sched,
                                                         const uint32_t* __restrict__ inv,
                                                         const uint32_t* __restrict__ nbr,
@@ -712,29 +1008,109 @@ __global__ __launch_bounds__(B) void k_list_gather(P pol, Sph s0, Scenes sc, uin
         if constexpr (kBatch) {
             scene = sc.sid[t];
             const Sph s = sc.tab[scene];
-            if (!(prev && prev[(size_t)scene * kMaxSolveIters] <= s.thr[kind]))
+            if (!(prev && prev[(size_t)scene * kMaxSolveIters] <= (kind ? s.thr[1] : s.thr[0])))
                 q = gather_one(pol, s, t, range, pos_b, pos_m, sched, inv, nbr, cnt, list_ok);
         } else {
             q = gather_one(pol, s0, t, range, pos_b, pos_m, sched, inv, nbr, cnt, list_ok);
         }
     }
-    if (partial) {
-        const unsigned long long b = block_sum_u64<B>(q);
-        if (threadIdx.x == 0) partial[blockIdx.x] = b;
-    }
-    if (kBatch && slot_out) {
-        // per-scene residual sums by atomics (integer, so order-independent): one per wave
-        // unless the wave straddles two scenes
-        const uint32_t s_first = __shfl(scene, 0, 64);
-        if (__all(scene == s_first)) {
-            unsigned long long v = q;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            if ((threadIdx.x & 63) == 0 && v) atomicAdd(slot_out + (size_t)s_first * kMaxSolveIters, v);
-        } else if (q) {
-            atomicAdd(slot_out + (size_t)scene * kMaxSolveIters, q);
+    sweep_epilogue<kBatch, B>(q, scene, partial, slot_out);
+}
+
+// The staged sweep of one window (one wave): the window's staging slots are loaded from the
+// schedule-ordered mirrors into LDS, then each lane walks its 16-bit list over LDS.
+// run: this lane's particle is swept (in range, its scene not converged).
+template <class P>
+__device__ __forceinline__ unsigned long long sweep_window(
+    const P& pol, const Sph& s, bool run, uint32_t t, uint32_t win, uint32_t last, float4* st, float2* st2, uint32_t* dl,
+    const uint2* __restrict__ range, const float4* __restrict__ pos_b, const float4* pos_m,
+    const uint32_t* __restrict__ sched, const uint32_t* __restrict__ inv, const uint32_t* __restrict__ nbr,
+    const uint32_t* __restrict__ cnt, const uint16_t* __restrict__ nbr16, const uint32_t* __restrict__ win_meta,
+    const uint4* __restrict__ win_round, const uint32_t* __restrict__ win_delta) {
+    if (!__any(run)) return 0;
+    const int lane = threadIdx.x & 63;
+    const uint32_t meta = win_meta[win];
+    if (meta == kWinFallback) return run ? gather_one(pol, s, t, range, pos_b, pos_m, sched, inv, nbr, cnt, 1u) : 0ull;
+    // the clamps keep every access in bounds whatever the tables hold
+    const uint32_t K = min(meta & 0xFFFFu, kWinMaxK), R = min(meta >> 16, kWinMaxRuns);
+    for (uint32_t r = lane; r < R; r += 64) dl[r] = win_delta[(size_t)win * kWinMaxRuns + r];
+    wave_sync();
+    const uint64_t below = (2ull << lane) - 1ull;   // bits 0..lane
+    for (uint32_t k = 0; k * 64u < K; k++) {
+        const uint4 rd = win_round[(size_t)win * kWinRounds + k];
+        const uint64_t mk = (uint64_t)rd.x | ((uint64_t)rd.y << 32);
+        const uint32_t slot = k * 64u + (uint32_t)lane;
+        if (slot < K) {
+            const uint32_t r = min(rd.z + (uint32_t)__popcll(mk & below), kWinMaxRuns - 1u);
+            pol.stage(st, st2, slot, min(slot + dl[r], last), pos_m);
         }
     }
+    wave_sync();
+    if (!run) return 0;
+    const uint32_t i = sched[t];
+    const float4 pi = geo_of(pol, pos_m, t);
+    const typename P::Self si = pol.self(i, t, pi);
+    typename P::Acc acc = pol.zero();
+    const uint32_t m = min(cnt[t], kMaxNbr);   // a window holds no overflowing list
+    const uint4* tile = list16_tile(nbr16, t);
+    for (uint32_t k = 0; k < m; k += 8) {
+        const uint4 qv = tile[(k >> 3) * 16];
+        const uint32_t w4[4] = {qv.x, qv.y, qv.z, qv.w};
+        float4 e[8];
+        typename P::Cand cj[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t loc = ((w4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu) & (kWinMaxK - 1u);
+            e[u] = st[loc];
+            cj[u] = pol.staged(st2, loc, e[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (k + u < m) pol.pair(acc, si, s, pi.x - e[u].x, pi.y - e[u].y, pi.z - e[u].z, cj[u]);
+    }
+    return pol.finish(i, t, acc, si, s);
+}
+
+template <class P, bool kBatch, int kWaves>
+__global__ __launch_bounds__(64 * kWaves) void k_sweep_win(P pol, Sph s0, Scenes sc, uint32_t n,
+                                                         const uint2* __restrict__ range,
+                                                         const float4* __restrict__ pos_b,
+                                                         const float4* pos_m,
+                                                         const uint32_t* __restrict__ sched,
+                                                         const uint32_t* __restrict__ inv,
+                                                         const uint32_t* __restrict__ nbr,
+                                                         const uint32_t* __restrict__ cnt,
+                                                         const uint16_t* __restrict__ nbr16,
+                                                         const uint32_t* __restrict__ win_meta,
+                                                         const uint4* __restrict__ win_round,
+                                                         const uint32_t* __restrict__ win_delta,
+                                                         unsigned long long* __restrict__ partial,
+                                                         const unsigned long long* __restrict__ prev,
+                                                         unsigned long long thr, uint32_t kind,
+                                                         unsigned long long* __restrict__ slot_out) {
+    constexpr int B = 64 * kWaves;
+    __shared__ float4 stage[kWaves][kWinMaxK];
+    __shared__ float2 stage2[kWaves][P::kParts > 1 ? kWinMaxK : 1];
+    __shared__ uint32_t sdelta[kWaves][kWinMaxRuns];
+    if (!kBatch && prev && *prev <= thr) return;
+    const int wv = threadIdx.x >> 6;
+    const uint32_t win = __builtin_amdgcn_readfirstlane(xcd_block() * kWaves + wv);
+    const uint32_t t = win * kWin + (threadIdx.x & 63);
+    unsigned long long q = 0;
+    uint32_t scene = 0;
+    if constexpr (kBatch) {
+        scene = t < n ? sc.sid[t] : 0u;
+        const Sph s = sc.tab[scene];
+        const bool run = t < n && !(prev && prev[(size_t)scene * kMaxSolveIters] <= (kind ? s.thr[1] : s.thr[0]));
+        q = sweep_window(pol, s, run, t, win, n - 1u, stage[wv], stage2[wv], sdelta[wv], range, pos_b, pos_m, sched, inv,
+                         nbr, cnt,
+                         nbr16, win_meta, win_round, win_delta);
+    } else {
+        q = sweep_window(pol, s0, t < n, t, win, n - 1u, stage[wv], stage2[wv], sdelta[wv], range, pos_b, pos_m, sched,
+                         inv, nbr,
+                         cnt, nbr16, win_meta, win_round, win_delta);
+    }
+    sweep_epilogue<kBatch, B>(q, scene, partial, slot_out);
 }
 
 // Sum of per-block residual partials into one sweep slot (single block, deterministic).
@@ -765,8 +1141,8 @@ hipError_t need(fe_ctx* c, uint32_t bits) {
     if ((c->mirrors & bits) == bits) return hipSuccess;
     Launch L(c, "refresh_mirrors");
     hipLaunchKernelGGL(k_refresh_mirrors, grid_for(c->n), dim3(kBlock), 0, c->stream, c->sched[0], c->n, c->pos_b,
-                       c->vel_b, c->vel_a, c->dens, c->pres, c->pacc, c->inv, c->pos_m, c->vb_m, c->va_m, c->dens_m,
-                       c->prho_m, c->pacc_m);
+                       c->vel_b, c->vel_a, c->dens, c->pres, c->pacc, c->fac, c->src, c->inv, c->pos_m, c->vb_m,
+                       c->va_m, c->dens_m, c->pa_m, c->pa2_m, c->fac_m, c->src_m, c->pres_m);
     c->mirrors = kMirrorAll;
     return hipGetLastError();
 }
@@ -805,6 +1181,54 @@ hipError_t run_list(fe_ctx* c, const char* name, int tid, const P& pol, unsigned
     return hipGetLastError();
 }
 
+template <class P, bool kBatch, int W>
+void launch_win(fe_ctx* c, const P& pol, unsigned long long* partial, const unsigned long long* prev,
+                unsigned long long thr, uint32_t kind, unsigned long long* slot_out) {
+    const uint32_t nwin = (c->n + kWin - 1) / kWin;
+    hipLaunchKernelGGL((k_sweep_win<P, kBatch, W>), dim3((nwin + W - 1) / W), dim3(64 * W), 0, c->stream, pol,
+                       sph_of(c), scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr,
+                       c->nbr_count, c->nbr16, c->win_meta, c->win_round, c->win_delta, partial, prev, thr, kind,
+                       slot_out);
+}
+
+template <class P, bool kBatch>
+void launch_win_b(fe_ctx* c, uint32_t block, const P& pol, unsigned long long* partial,
+                  const unsigned long long* prev, unsigned long long thr, uint32_t kind,
+                  unsigned long long* slot_out) {
+    switch (block) {
+        case 64: launch_win<P, kBatch, 1>(c, pol, partial, prev, thr, kind, slot_out); break;
+        case 128: launch_win<P, kBatch, 2>(c, pol, partial, prev, thr, kind, slot_out); break;
+        default: launch_win<P, kBatch, 4>(c, pol, partial, prev, thr, kind, slot_out); break;
+    }
+}
+
+// A Jacobi sweep: staged windows over the current list (tuned items 2), or per-particle
+// gathers of the listed pairs (items 1; the bucket walk when there is no current list).
+template <class P>
+hipError_t run_sweep(fe_ctx* c, const char* name, int tid, const P& pol, unsigned long long* partial,
+                     const unsigned long long* prev, unsigned long long thr, uint32_t kind,
+                     unsigned long long* slot_out) {
+    if (!c->list_valid || c->tune[tid].items != 2) return run_list(c, name, tid, pol, partial, prev, thr, kind, slot_out);
+    if (c->win_gen != c->list_gen) {   // the current list has no window tables yet
+        Launch W(c, "window_tables");
+        constexpr int kTabWaves = 4;
+        const uint32_t nwin = (c->n + kWin - 1) / kWin;
+        hipLaunchKernelGGL(k_window_tables<kTabWaves>, dim3((nwin + kTabWaves - 1) / kTabWaves), dim3(64 * kTabWaves),
+                           0, c->stream, c->n, c->nbr, c->nbr_count, c->win_meta, c->win_round, c->win_delta,
+                           c->nbr16);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        c->win_gen = c->list_gen;
+    }
+    Launch L(c, name);
+    const uint32_t block = c->tune[tid].block;
+    if (c->nscenes > 1)
+        launch_win_b<P, true>(c, block, pol, partial, prev, thr, kind, slot_out);
+    else
+        launch_win_b<P, false>(c, block, pol, partial, prev, thr, kind, slot_out);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 // Residual partials: one per block of the smallest gather block size.
@@ -813,8 +1237,8 @@ uint32_t gather_partials(uint32_t n) { return (n + 63) / 64; }
 hipError_t launch_schedule_mirrors(fe_ctx* c) {
     Launch L(c, "schedule_mirrors");
     hipLaunchKernelGGL(k_schedule_mirrors, grid_for(c->n), dim3(kBlock), 0, c->stream, c->sched[0], c->n, c->pos_b,
-                       c->vel_b, c->inv, c->pos_m, c->vb_m);
-    c->mirrors = kMirrorPos | kMirrorVelB;
+                       c->vel_b, c->vel_a, c->inv, c->pos_m, c->vb_m, c->va_m);
+    c->mirrors = kMirrorPos | kMirrorVelB | kMirrorVelA;
     c->list_valid = false;
     return hipGetLastError();
 }
@@ -845,14 +1269,17 @@ hipError_t launch_build_list(fe_ctx* c) {
     }
 #undef FE_BUILD
     e = hipGetLastError();
-    if (e == hipSuccess) c->list_valid = true;
+    if (e == hipSuccess) {
+        c->list_valid = true;
+        c->list_gen++;   // window tables are built from it on first use
+    }
     return e;
 }
 
 hipError_t launch_density_alpha_only(fe_ctx* c) {
-    DensityAlpha pol{c->dens, c->fac, c->dens_m};
+    DensityAlpha pol{c->dens, c->fac, c->dens_m, c->fac_m};
     hipError_t e = run_list(c, "density_alpha", kTuneDensityAlpha, pol);
-    c->mirrors = (c->mirrors | kMirrorDens) & ~kMirrorPrho;   // p / rho^2 used the old densities
+    c->mirrors = (c->mirrors | kMirrorDens | kMirrorFac) & ~kMirrorPrho;   // p / rho^2 used the old densities
     c->da_fresh = c->ns_fresh;
     return e;
 }
@@ -874,7 +1301,7 @@ hipError_t launch_viscosity(fe_ctx* c) {
 hipError_t launch_density_source(fe_ctx* c) {
     hipError_t e = need(c, kMirrorPos | kMirrorVelA);
     if (e != hipSuccess) return e;
-    DensitySource pol{c->va_m, c->dens, c->src, c->pres, c->prho_m};
+    DensitySource pol{c->va_m, c->dens, c->src, c->pres, c->pos_m, c->src_m, c->pres_m};
     if (learn_features_wanted(c, 0)) {
         if ((e = learn_prev(c, 0)) != hipSuccess) return e;
         DensitySourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n};
@@ -883,14 +1310,14 @@ hipError_t launch_density_source(fe_ctx* c) {
     } else {
         e = run_list(c, "density_source", kTuneDensitySource, pol);
     }
-    c->mirrors |= kMirrorPrho;
+    c->mirrors |= kMirrorPrho | kMirrorSrc | kMirrorPres;
     return e;
 }
 
 hipError_t launch_divergence_source(fe_ctx* c) {
-    hipError_t e = need(c, kMirrorPos);
+    hipError_t e = need(c, kMirrorPos | kMirrorVelA);
     if (e != hipSuccess) return e;
-    DivergenceSource pol{c->vel_a, c->sched[0], c->dens, c->pres, c->src, c->prho_m};
+    DivergenceSource pol{c->va_m, c->dens, c->pres, c->src, c->pos_m, c->src_m, c->pres_m};
     if (learn_features_wanted(c, 1)) {
         if ((e = learn_prev(c, 1)) != hipSuccess) return e;
         DivergenceSourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n};
@@ -899,27 +1326,28 @@ hipError_t launch_divergence_source(fe_ctx* c) {
     } else {
         e = run_list(c, "divergence_source", kTuneDivergenceSource, pol);
     }
-    c->mirrors |= kMirrorPrho;
+    c->mirrors |= kMirrorPrho | kMirrorSrc | kMirrorPres;
     return e;
 }
 
 hipError_t launch_pressure_force(fe_ctx* c, const unsigned long long* prev, unsigned long long thr, uint32_t kind) {
     hipError_t e = need(c, kMirrorPos | kMirrorPrho);
     if (e != hipSuccess) return e;
-    PressureForce pol{c->pres, c->dens, c->prho_m, c->pacc, c->pacc_m};
-    e = run_list(c, "pressure_force", kTunePressureForce, pol, nullptr, prev, thr, kind);
+    PressureForce pol{c->pacc, c->pa_m, c->pa2_m};
+    e = run_sweep(c, "pressure_force", kTunePressureForce, pol, nullptr, prev, thr, kind, nullptr);
     c->mirrors |= kMirrorPacc;
     return e;
 }
 
 hipError_t launch_pressure_update(fe_ctx* c, unsigned long long* out, const unsigned long long* prev,
                                   unsigned long long thr, uint32_t kind) {
-    hipError_t e = need(c, kMirrorPos | kMirrorPacc | kMirrorPrho);
+    hipError_t e = need(c, kMirrorPacc | kMirrorDens | kMirrorFac | kMirrorSrc | kMirrorPres);
     if (e != hipSuccess) return e;
-    PressureUpdate pol{c->pacc_m, c->fac, c->src, c->dens, c->pres, c->prho_m};
+    PressureUpdate pol{c->pa_m, c->pa2_m, c->fac_m, c->src_m, c->dens_m, c->pres, c->pres_m, c->pos_m};
     if (c->nscenes > 1)   // per-scene sums straight into the scenes' slots
-        return run_list(c, "pressure_update", kTunePressureUpdate, pol, nullptr, prev, thr, kind, out);
-    e = run_list(c, "pressure_update", kTunePressureUpdate, pol, out ? c->partial : nullptr, prev, thr, kind);
+        return run_sweep(c, "pressure_update", kTunePressureUpdate, pol, nullptr, prev, thr, kind, out);
+    e = run_sweep(c, "pressure_update", kTunePressureUpdate, pol, out ? c->partial : nullptr, prev, thr, kind,
+                  nullptr);
     if (e != hipSuccess || !out) return e;
     Launch L(c, "resid_finalize");
     const uint32_t pb = c->tune[kTunePressureUpdate].block;

@@ -17,9 +17,19 @@ constexpr uint32_t kBlock = 256;          // threads per block of the per-partic
 constexpr uint32_t kMaxSolveIters = 1024; // residual slots per solve
 constexpr uint32_t kStatsBlocks = 1024;   // blocks of the stats reduction
 constexpr uint32_t kMaxNbr = 64;          // neighbour-list slots per particle
-// schedule-ordered mirrors of the fields the gathers read from neighbours
+// schedule-ordered mirrors of the fields the gathers read from neighbours, and of the
+// per-particle solver fields the Jacobi sweeps read for their own particle
 constexpr uint32_t kMirrorPos = 1u, kMirrorVelB = 2u, kMirrorVelA = 4u, kMirrorDens = 8u, kMirrorPrho = 16u,
-                   kMirrorPacc = 32u, kMirrorAll = 63u;
+                   kMirrorPacc = 32u, kMirrorFac = 64u, kMirrorSrc = 128u, kMirrorPres = 256u,
+                   kMirrorAll = 511u;
+// Windows of the Jacobi sweeps: 64 consecutive schedule positions (one wave). The union of
+// the window's listed neighbours is staged in LDS once per sweep, as at most kWinMaxRuns
+// runs of consecutive schedule positions holding at most kWinMaxK entries in total.
+constexpr uint32_t kWin = 64;
+constexpr uint32_t kWinMaxK = 512;
+constexpr uint32_t kWinRounds = kWinMaxK / kWin;   // 64-entry staging rounds
+constexpr uint32_t kWinMaxRuns = 192;
+constexpr uint32_t kWinFallback = 0xFFFFFFFFu;     // window meta: use the per-particle list
 
 // Per-kernel event profiling (enabled by fe_profile_enable).
 struct ProfileRecord {
@@ -172,15 +182,33 @@ struct fe_ctx {
     uint32_t* sched[2] = {nullptr, nullptr};
     uint32_t* mkeys[2] = {nullptr, nullptr};
     uint32_t morton_bits = 30;   // bits of the Morton key (3 x bits per axis)
+    uint32_t exact_fast = 0;     // exact arithmetic shortcuts valid for params (kFast* bits)
     uint32_t scene_bits = 0;     // bits of the scene number above them (a batch)
     uint32_t* inv = nullptr;     // inverse schedule: hash index -> schedule position
     float4* pos_m = nullptr;     // position_b by schedule position
     float4* vb_m = nullptr;      // velocity_b
     float4* va_m = nullptr;      // velocity_a
     float* dens_m = nullptr;     // densities
-    float* prho_m = nullptr;     // pressure / density^2
-    float4* pacc_m = nullptr;    // pressure_accelerations
+    // pos_m.w holds pressure / density^2 (kMirrorPrho); pa_m = (position_b xyz, pressure
+    // acceleration x) and pa2_m = (pressure acceleration y, z) (kMirrorPacc), so a sweep
+    // gathers one 16-byte record (pressure_force) or 16 + 8 bytes (pressure_update) per pair
+    float4* pa_m = nullptr;
+    float2* pa2_m = nullptr;
+    float* fac_m = nullptr;      // factors
+    float* src_m = nullptr;      // source_terms
+    float* pres_m = nullptr;     // pressures
     uint32_t mirrors = 0;        // kMirror* bits that are current
+    // sweep windows of the current list (valid with it): per window the staged entry count K
+    // and run count R (K | R << 16, or kWinFallback), per staging round the 64-bit mask of
+    // runs that start inside the round and the run holding its first entry, per run the
+    // offset from staging slot to schedule position, and the list as 16-bit staging slots
+    // (tiles of 16 work-items x 8 slots)
+    uint64_t list_gen = 0;           // lists built so far
+    uint64_t win_gen = ~0ull;        // the list the window tables were built from
+    uint32_t* win_meta = nullptr;
+    uint4* win_round = nullptr;
+    uint32_t* win_delta = nullptr;
+    uint16_t* nbr16 = nullptr;
     // The neighbour structure and densities still describe position_a (nothing moved the
     // particles since the last neighbour search + density_alpha): the next frame-start
     // search would reproduce them bit for bit, so only velocity_b is re-gathered.
@@ -252,6 +280,10 @@ hipError_t launch_schedule_mirrors(fe_ctx* c);
 hipError_t launch_reorder_velocity(fe_ctx* c);
 
 // ---- launchers (fe_sph.hip) ----
+// The exact arithmetic shortcuts (kFast* bits) valid for a scene with parameters p: the
+// constant-divisor quotient is checked on the device once per smoothing radius (cached per
+// process), the operand box of the gradient division from h on the host.
+hipError_t exact_math_flags(fe_ctx* c, const fe_params& p, uint32_t* flags);
 hipError_t launch_hash(fe_ctx* c);
 hipError_t launch_offsets_reorder(fe_ctx* c);
 // Morton keys of the cells of position_b and the identity schedule payload.

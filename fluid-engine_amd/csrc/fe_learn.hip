@@ -276,13 +276,14 @@ __device__ __forceinline__ void store_act(const float2* lut, const float* sb, co
 
 // Per sample s < n (s = schedule position): y = MLP(standardised features). With raw != 0,
 // y goes to yraw[s] and the inputs are used as given; otherwise p = max(0, y target_std +
-// target_mean) goes to pressures[sched[s]] and p / rho^2 to the mirror prho_m[s].
+// target_mean) goes to pressures[sched[s]], to its mirror pres_m[s], and p / rho^2 to the
+// w of the position mirror pos_m[s].
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ packed,
                                                    const float2* __restrict__ gelu_table,
                                                    const float* __restrict__ feat, uint32_t n, uint32_t raw,
                                                    const uint32_t* __restrict__ sched, const float* __restrict__ dens,
-                                                   float* __restrict__ pres, float* __restrict__ prho_m,
-                                                   float* __restrict__ yraw) {
+                                                   float* __restrict__ pres, float4* __restrict__ pos_m,
+                                                   float* __restrict__ pres_m, float* __restrict__ yraw) {
     __shared__ alignas(16) float sf[PK_END - PK_B1];
     __shared__ alignas(16) __bf16 act[kInferWaves][16 * H];
     __shared__ float2 lut[kGeluNodes];
@@ -367,7 +368,8 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ pac
                 float p = y * tstd + tmean;
                 p = p > 0.0f ? p : 0.0f;
                 pres[i] = p;
-                prho_m[s] = prho_of(p, dens[i]);
+                pres_m[s] = p;
+                pos_m[s].w = prho_of(p, dens[i]);
             }
         }
     }
@@ -622,7 +624,7 @@ hipError_t learn_infer(fe_ctx* c, int which) {
     Launch L(c, "ws_infer");
     hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(c->n)), dim3(64 * kInferWaves), 0, c->stream,
                        c->ws.packed[which], gelu_table_of(c, which), c->ws.feat, c->n, 0u, c->sched[0], c->dens,
-                       c->pres, c->prho_m, (float*)nullptr);
+                       c->pres, c->pos_m, c->pres_m, (float*)nullptr);
     return hipGetLastError();
 }
 
@@ -643,7 +645,7 @@ hipError_t learn_forward_raw(fe_ctx* c, int which, const float* x, uint32_t n, f
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(n)), dim3(64 * kInferWaves), 0, c->stream, c->ws.packed[which],
                        gelu_table_of(c, which), x, n, 1u, (const uint32_t*)nullptr, (const float*)nullptr,
-                       (float*)nullptr, (float*)nullptr, y);
+                       (float*)nullptr, (float4*)nullptr, (float*)nullptr, y);
     return hipGetLastError();
 }
 

@@ -7,6 +7,12 @@
 // pressure accelerations and colours are float4[N]; densities, alpha factors, source terms
 // and pressures are float[N]; the sorted entries are two uint32[T] arrays (hash, index) and
 // each hash bucket's [start, end) is one uint2 in `range` (start = grid_start).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+
 #include "fe_device.h"
 
 namespace fe {
@@ -406,6 +412,87 @@ hipError_t launch_offsets_reorder(fe_ctx* c) {
     return hipGetLastError();
 }
 
+namespace {
+
+// Whether the residual-corrected product equals the IEEE quotient r / h for every float r in
+// [lo, lo + count): any mismatch sets *bad.
+__global__ __launch_bounds__(kBlock) void k_check_div_h(float h, float inv_h, uint32_t lo, uint32_t count,
+                                                        uint32_t* __restrict__ bad) {
+    const uint32_t stride = gridDim.x * kBlock;
+    uint32_t miss = 0;
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count; k += stride) {
+        const float r = __uint_as_float(lo + k);
+        const float q = r / h;
+        const float q0 = r * inv_h;
+        const float e = fmaf(-q0, h, r);
+        const float f = fmaf(e, inv_h, q0);
+        miss |= __float_as_uint(q) != __float_as_uint(f);
+    }
+    if (__any(miss) && (threadIdx.x & 63) == 0) atomicOr(bad, 1u);
+}
+
+// The gradient division gf / (h r) over the pairs a scene can produce, 1e-6 <= r <= 1.001 h:
+// h r within [1e-6 h, 1.001 h^2], |gf| within [min(3 k 1e-6 / h, 6 k 2^-48), 2 k] with
+// k = 8 / (pi h^3), all inside [2^-40, 2^40) with a factor-4 margin.
+bool div_box_ok(float h) {
+    const double hd = h, k = 8.0 / (3.14159265358979323846 * hd * hd * hd);
+    const double lo = std::ldexp(1.0, -38), hi = std::ldexp(1.0, 38);
+    const double vals[] = {1e-6 * hd, 1.001 * hd * hd, std::min(3.0 * k * 1e-6 / hd, 6.0 * k * std::ldexp(1.0, -48)),
+                           2.0 * k};
+    for (double v : vals)
+        if (!(v >= lo && v <= hi)) return false;
+    return true;
+}
+
+std::mutex g_div_h_mutex;
+std::unordered_map<uint32_t, bool> g_div_h_ok;   // by the bits of h
+
+}  // namespace
+
+hipError_t exact_math_flags(fe_ctx* c, const fe_params& p, uint32_t* flags) {
+    const float h = p.smoothing_radius;
+    uint32_t hb;
+    std::memcpy(&hb, &h, 4);
+    bool ok_h = false, known = false;
+    {
+        std::lock_guard<std::mutex> lk(g_div_h_mutex);
+        auto it = g_div_h_ok.find(hb);
+        if (it != g_div_h_ok.end()) {
+            ok_h = it->second;
+            known = true;
+        }
+    }
+    if (!known) {
+        const float lo_f = 1e-6f, hi_f = h * 1.001f;
+        uint32_t lo, hi;
+        std::memcpy(&lo, &lo_f, 4);
+        std::memcpy(&hi, &hi_f, 4);
+        if (!(h > 1e-6f) || !std::isfinite(hi_f)) {
+            ok_h = false;
+        } else {
+            uint32_t* bad = nullptr;
+            hipError_t e = hipMalloc((void**)&bad, sizeof(uint32_t));
+            if (e != hipSuccess) return e;
+            uint32_t host_bad = 1u;
+            e = hipMemsetAsync(bad, 0, sizeof(uint32_t), c->stream);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_check_div_h, dim3(2048), dim3(kBlock), 0, c->stream, h, 1.0f / h, lo, hi - lo + 1u,
+                                   bad);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipMemcpyAsync(&host_bad, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            (void)hipFree(bad);
+            if (e != hipSuccess) return e;
+            ok_h = host_bad == 0u;
+        }
+        std::lock_guard<std::mutex> lk(g_div_h_mutex);
+        g_div_h_ok[hb] = ok_h;
+    }
+    *flags = (ok_h ? kFastDivH : 0u) | (div_box_ok(h) ? kFastDivBox : 0u);
+    return hipSuccess;
+}
+
 hipError_t launch_morton_keys(fe_ctx* c) {
     Launch L(c, "morton_keys");
     hipLaunchKernelGGL(k_morton_keys, grid_for(c->n), dim3(kBlock), 0, c->stream, c->pos_b, c->n, sph_of(c),
@@ -516,6 +603,10 @@ bool tune_valid(int id, uint32_t block, uint32_t items) {
     if (id == kTuneBuildList) return (block == 64 || block == 128 || block == 256) && items == 1;
     if (id == kTunePressureIntegration || id == kTuneDivergenceIntegration)
         return pow2 && (items == 1 || items == 2 || items == 4);
+    // the sweeps: items 1 gathers each listed pair from the mirrors, items 2 stages one window
+    // per wave in LDS (1, 2 or 4 windows per block)
+    if (id == kTunePressureForce || id == kTunePressureUpdate)
+        return (pow2 && items == 1) || ((block == 64 || block == 128 || block == 256) && items == 2);
     return id >= 0 && id < kTuneCount && pow2 && items == 1;
 }
 

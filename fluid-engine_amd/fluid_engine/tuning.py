@@ -2,7 +2,7 @@
 
 Entries are keyed by device ("<gcn arch>/<compute units>cu"), particle count and scene count,
 so a configuration measured once on a device is reused by later runs of the same workload.
-File format: {"version": 1, "entries": {key: {kernel: {"block", "items", "median_us",
+File format: {"version": 2, "entries": {key: {kernel: {"block", "items", "median_us",
 "default_us"}}}}.
 """
 from __future__ import annotations
@@ -10,7 +10,9 @@ from __future__ import annotations
 import json
 import os
 
-VERSION = 1
+from ._ffi import FluidEngineError
+
+VERSION = 2
 
 
 def cache_key(device: str, particles: int, scenes: int = 1) -> str:
@@ -42,9 +44,12 @@ def autotune(sim, cache_path: str | None = None, reps: int = 20, retune: bool = 
     cache = load_cache(cache_path)
     entry = cache["entries"].get(key)
     if entry and not retune:
-        for kernel, cfg in entry.items():
-            sim.set_tune(kernel, int(cfg["block"]), int(cfg["items"]))
-        return entry
+        try:
+            for kernel, cfg in entry.items():
+                sim.set_tune(kernel, int(cfg["block"]), int(cfg["items"]))
+            return entry
+        except FluidEngineError:   # a configuration this library no longer builds: measure again
+            pass
     entry = sim.autotune_now(reps)
     if cache_path:
         cache["entries"][key] = entry

@@ -61,7 +61,7 @@ def test_explicit_configurations_same_fields():
     a = fe.Simulation(w.params, w.positions, device=0, options=conv())
     b = fe.Simulation(w.params, w.positions, device=0, options=conv())
     try:
-        for k, blk, it in (("pressure_force", 64, 1), ("pressure_update", 1024, 1), ("density_alpha", 512, 1),
+        for k, blk, it in (("pressure_force", 64, 2), ("pressure_update", 1024, 1), ("density_alpha", 512, 1),
                            ("viscosity", 128, 1), ("build_list", 64, 1), ("pressure_integration", 1024, 4),
                            ("divergence_integration", 64, 2), ("density_source", 1024, 1),
                            ("divergence_source", 64, 1)):
@@ -75,6 +75,8 @@ def test_explicit_configurations_same_fields():
             b.set_tune("pressure_force", 96, 1)
         with pytest.raises(fe.FluidEngineError):
             b.set_tune("build_list", 512, 1)
+        with pytest.raises(fe.FluidEngineError):   # staged sweeps run 1, 2 or 4 windows per block
+            b.set_tune("pressure_update", 512, 2)
         with pytest.raises(fe.FluidEngineError):
             b.set_tune("no_such_kernel", 256, 1)
     finally:

@@ -4,6 +4,7 @@ damaged or foreign file is ignored."""
 import json
 
 from fluid_engine import tuning
+from fluid_engine._ffi import FluidEngineError
 
 
 class FakeSim:
@@ -21,6 +22,8 @@ class FakeSim:
         return {"pressure_force": {"block": 128, "items": 1, "median_us": 10.0, "default_us": 12.0}}
 
     def set_tune(self, kernel, block, items):
+        if block not in (64, 128, 256, 512, 1024):
+            raise FluidEngineError(1, "fe_tune_set", "no build for block %d" % block)
         self.applied[kernel] = (block, items)
 
 
@@ -30,7 +33,7 @@ def test_cache_roundtrip(tmp_path):
     r = tuning.autotune(a, path, reps=3)
     assert a.measured == 1 and r["pressure_force"]["block"] == 128
     data = json.load(open(path))
-    assert data["version"] == 1
+    assert data["version"] == tuning.VERSION
     assert tuning.cache_key("gfx950/256cu", 1000, 1) in data["entries"]
     b = FakeSim()
     tuning.autotune(b, path)
@@ -50,4 +53,16 @@ def test_bad_cache_is_ignored(tmp_path):
     s = FakeSim()
     tuning.autotune(s, str(path))
     assert s.measured == 1
-    assert json.load(open(path))["version"] == 1
+    assert json.load(open(path))["version"] == tuning.VERSION
+
+
+def test_stale_entry_is_measured_again(tmp_path):
+    """An entry naming a configuration the library no longer builds is re-measured."""
+    path = str(tmp_path / "tune.json")
+    key = tuning.cache_key("gfx950/256cu", 1000, 1)
+    stale = {"pressure_force": {"block": 96, "items": 1, "median_us": 1.0, "default_us": 1.0}}
+    tuning.save_cache(path, {"version": tuning.VERSION, "entries": {key: stale}})
+    s = FakeSim()
+    r = tuning.autotune(s, path)
+    assert s.measured == 1 and r["pressure_force"]["block"] == 128
+    assert json.load(open(path))["entries"][key]["pressure_force"]["block"] == 128
