@@ -60,7 +60,7 @@ void free_ctx(fe_ctx* c) {
                     (void*)c->pacc, (void*)c->dens, (void*)c->fac, (void*)c->src, (void*)c->pres,
                     (void*)c->keys[0], (void*)c->keys[1], (void*)c->vals[0], (void*)c->vals[1],
                     (void*)c->range, (void*)c->stats, (void*)c->resid, (void*)c->staging,
-                    (void*)c->partial, (void*)c->stats_partial, (void*)c->nbr, (void*)c->nbr_count,
+                    (void*)c->stats_partial, (void*)c->nbr, (void*)c->nbr_count,
                     (void*)c->sched[0], (void*)c->sched[1], (void*)c->mkeys[0], (void*)c->mkeys[1],
                     (void*)c->inv, (void*)c->pos_m, (void*)c->vb_m, (void*)c->va_m, (void*)c->dens_m,
                     (void*)c->pa_m, (void*)c->pa2_m, (void*)c->fac_m, (void*)c->src_m, (void*)c->pres_m,
@@ -480,7 +480,6 @@ static fe_status alloc_ctx(fe_ctx* c, const float* xyz) {
     ALLOC(c->range, 2 * tb);
     ALLOC(c->stats, 16);
     ALLOC(c->resid, sizeof(unsigned long long) * slots);
-    ALLOC(c->partial, sizeof(unsigned long long) * fe::gather_partials(n));
     ALLOC(c->stats_partial, sizeof(uint32_t) * 3 * fe::kStatsBlocks);
     ALLOC(c->nbr, sizeof(uint32_t) * fe::kMaxNbr * (((size_t)n + 15) / 16 * 16));   // whole 16-item tiles
     ALLOC(c->nbr_count, sizeof(uint32_t) * (size_t)n);

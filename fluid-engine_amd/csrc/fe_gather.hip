@@ -915,18 +915,17 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v
     return t;
 }
 
-// A sweep block's residual output: the block's sum into partial[blockIdx.x] (one scene), or
-// per-scene sums by atomics into the scenes' slots (integer, so order-independent; one per
-// wave unless the wave straddles two scenes).
+// A sweep block's residual output, summed into the sweep's slot by integer atomics (so the
+// order does not matter): one per block for one scene; for a batch one per wave into its
+// scene's slot, unless the wave straddles two scenes.
 template <bool kBatch, int B>
 __device__ __forceinline__ void sweep_epilogue(unsigned long long q, uint32_t scene,
-                                               unsigned long long* __restrict__ partial,
                                                unsigned long long* __restrict__ slot_out) {
-    if (partial) {
+    if (!slot_out) return;
+    if constexpr (!kBatch) {
         const unsigned long long b = block_sum_u64<B>(q);
-        if (threadIdx.x == 0) partial[blockIdx.x] = b;
-    }
-    if (kBatch && slot_out) {
+        if (threadIdx.x == 0 && b) atomicAdd(slot_out, b);
+    } else {
         const uint32_t s_first = __shfl(scene, 0, 64);
         if (__all(scene == s_first)) {
             unsigned long long v = q;
@@ -939,11 +938,8 @@ __device__ __forceinline__ void sweep_epilogue(unsigned long long q, uint32_t sc
     }
 }
 
-// One work-item per particle: the listed pairs, four neighbour loads in flight, or the
-// bucket walk when the list is stale or the particle overflowed it. With `prev` set, a
-// converged previous sweep skips the launch; with `partial` set, each block stores the sum
-// of its particles' finish() values.
-// One work-item's gather: the listed pairs of schedule position t (or the bucket walk).
+// One work-item's gather: the listed pairs of schedule position t, four neighbour loads in
+// flight, or the bucket walk when the list is stale or the particle overflowed it.
 template <class P>
 __device__ __forceinline__ unsigned long long gather_one(const P& pol, const Sph& s, uint32_t t,
                                                          const uint2* __restrict__ range,
@@ -996,7 +992,6 @@ __global__ __launch_bounds__(B) void k_list_gather(P pol, Sph s0, Scenes sc, uin
                                                         const uint32_t* __restrict__ inv,
                                                         const uint32_t* __restrict__ nbr,
                                                         const uint32_t* __restrict__ cnt, uint32_t list_ok,
-                                                        unsigned long long* __restrict__ partial,
                                                         const unsigned long long* __restrict__ prev,
                                                         unsigned long long thr, uint32_t kind,
                                                         unsigned long long* __restrict__ slot_out) {
@@ -1014,7 +1009,7 @@ __global__ __launch_bounds__(B) void k_list_gather(P pol, Sph s0, Scenes sc, uin
             q = gather_one(pol, s0, t, range, pos_b, pos_m, sched, inv, nbr, cnt, list_ok);
         }
     }
-    sweep_epilogue<kBatch, B>(q, scene, partial, slot_out);
+    sweep_epilogue<kBatch, B>(q, scene, slot_out);
 }
 
 // The staged sweep of one window (one wave): the window's staging slots are loaded from the
@@ -1084,7 +1079,6 @@ __global__ __launch_bounds__(64 * kWaves) void k_sweep_win(P pol, Sph s0, Scenes
                                                          const uint32_t* __restrict__ win_meta,
                                                          const uint4* __restrict__ win_round,
                                                          const uint32_t* __restrict__ win_delta,
-                                                         unsigned long long* __restrict__ partial,
                                                          const unsigned long long* __restrict__ prev,
                                                          unsigned long long thr, uint32_t kind,
                                                          unsigned long long* __restrict__ slot_out) {
@@ -1110,28 +1104,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_sweep_win(P pol, Sph s0, Scenes
                          inv, nbr,
                          cnt, nbr16, win_meta, win_round, win_delta);
     }
-    sweep_epilogue<kBatch, B>(q, scene, partial, slot_out);
-}
-
-// Sum of per-block residual partials into one sweep slot (single block, deterministic).
-__global__ __launch_bounds__(kBlock) void k_resid_finalize(const unsigned long long* __restrict__ partial,
-                                                           uint32_t count, unsigned long long* __restrict__ out,
-                                                           const unsigned long long* __restrict__ prev,
-                                                           unsigned long long thr) {
-    if (prev && *prev <= thr) return;
-    __shared__ unsigned long long part[kBlock / 64];
-    unsigned long long v = 0;
-    for (uint32_t k = threadIdx.x; k < count; k += kBlock) v += partial[k];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) part[w] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long tot = 0;
-        for (int k = 0; k < (int)(kBlock / 64); k++) tot += part[k];
-        *out = tot;
-    }
+    sweep_epilogue<kBatch, B>(q, scene, slot_out);
 }
 
 inline dim3 grid_for(uint32_t count) { return dim3((count + kBlock - 1) / kBlock); }
@@ -1148,67 +1121,63 @@ hipError_t need(fe_ctx* c, uint32_t bits) {
 }
 
 template <class P, bool kBatch, int B>
-void launch_gather(fe_ctx* c, const P& pol, unsigned long long* partial, const unsigned long long* prev,
+void launch_gather(fe_ctx* c, const P& pol, const unsigned long long* prev,
                    unsigned long long thr, uint32_t kind, unsigned long long* slot_out) {
     hipLaunchKernelGGL((k_list_gather<P, kBatch, B>), dim3((c->n + B - 1) / B), dim3(B), 0, c->stream, pol,
                        sph_of(c), scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr,
-                       c->nbr_count, c->list_valid ? 1u : 0u, partial, prev, thr, kind, slot_out);
+                       c->nbr_count, c->list_valid ? 1u : 0u, prev, thr, kind, slot_out);
 }
 
 template <class P, bool kBatch>
-void launch_gather_b(fe_ctx* c, uint32_t block, const P& pol, unsigned long long* partial,
-                     const unsigned long long* prev, unsigned long long thr, uint32_t kind,
+void launch_gather_b(fe_ctx* c, uint32_t block, const P& pol, const unsigned long long* prev, unsigned long long thr, uint32_t kind,
                      unsigned long long* slot_out) {
     switch (block) {
-        case 64: launch_gather<P, kBatch, 64>(c, pol, partial, prev, thr, kind, slot_out); break;
-        case 128: launch_gather<P, kBatch, 128>(c, pol, partial, prev, thr, kind, slot_out); break;
-        case 512: launch_gather<P, kBatch, 512>(c, pol, partial, prev, thr, kind, slot_out); break;
-        case 1024: launch_gather<P, kBatch, 1024>(c, pol, partial, prev, thr, kind, slot_out); break;
-        default: launch_gather<P, kBatch, 256>(c, pol, partial, prev, thr, kind, slot_out); break;
+        case 64: launch_gather<P, kBatch, 64>(c, pol, prev, thr, kind, slot_out); break;
+        case 128: launch_gather<P, kBatch, 128>(c, pol, prev, thr, kind, slot_out); break;
+        case 512: launch_gather<P, kBatch, 512>(c, pol, prev, thr, kind, slot_out); break;
+        case 1024: launch_gather<P, kBatch, 1024>(c, pol, prev, thr, kind, slot_out); break;
+        default: launch_gather<P, kBatch, 256>(c, pol, prev, thr, kind, slot_out); break;
     }
 }
 
 template <class P>
-hipError_t run_list(fe_ctx* c, const char* name, int tid, const P& pol, unsigned long long* partial = nullptr,
-                    const unsigned long long* prev = nullptr, unsigned long long thr = 0, uint32_t kind = 0,
+hipError_t run_list(fe_ctx* c, const char* name, int tid, const P& pol, const unsigned long long* prev = nullptr, unsigned long long thr = 0, uint32_t kind = 0,
                     unsigned long long* slot_out = nullptr) {
     Launch L(c, name);
     const uint32_t block = c->tune[tid].block;
     if (c->nscenes > 1)
-        launch_gather_b<P, true>(c, block, pol, partial, prev, thr, kind, slot_out);
+        launch_gather_b<P, true>(c, block, pol, prev, thr, kind, slot_out);
     else
-        launch_gather_b<P, false>(c, block, pol, partial, prev, thr, kind, slot_out);
+        launch_gather_b<P, false>(c, block, pol, prev, thr, kind, slot_out);
     return hipGetLastError();
 }
 
 template <class P, bool kBatch, int W>
-void launch_win(fe_ctx* c, const P& pol, unsigned long long* partial, const unsigned long long* prev,
+void launch_win(fe_ctx* c, const P& pol, const unsigned long long* prev,
                 unsigned long long thr, uint32_t kind, unsigned long long* slot_out) {
     const uint32_t nwin = (c->n + kWin - 1) / kWin;
     hipLaunchKernelGGL((k_sweep_win<P, kBatch, W>), dim3((nwin + W - 1) / W), dim3(64 * W), 0, c->stream, pol,
                        sph_of(c), scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr,
-                       c->nbr_count, c->nbr16, c->win_meta, c->win_round, c->win_delta, partial, prev, thr, kind,
+                       c->nbr_count, c->nbr16, c->win_meta, c->win_round, c->win_delta, prev, thr, kind,
                        slot_out);
 }
 
 template <class P, bool kBatch>
-void launch_win_b(fe_ctx* c, uint32_t block, const P& pol, unsigned long long* partial,
-                  const unsigned long long* prev, unsigned long long thr, uint32_t kind,
+void launch_win_b(fe_ctx* c, uint32_t block, const P& pol, const unsigned long long* prev, unsigned long long thr, uint32_t kind,
                   unsigned long long* slot_out) {
     switch (block) {
-        case 64: launch_win<P, kBatch, 1>(c, pol, partial, prev, thr, kind, slot_out); break;
-        case 128: launch_win<P, kBatch, 2>(c, pol, partial, prev, thr, kind, slot_out); break;
-        default: launch_win<P, kBatch, 4>(c, pol, partial, prev, thr, kind, slot_out); break;
+        case 64: launch_win<P, kBatch, 1>(c, pol, prev, thr, kind, slot_out); break;
+        case 128: launch_win<P, kBatch, 2>(c, pol, prev, thr, kind, slot_out); break;
+        default: launch_win<P, kBatch, 4>(c, pol, prev, thr, kind, slot_out); break;
     }
 }
 
 // A Jacobi sweep: staged windows over the current list (tuned items 2), or per-particle
 // gathers of the listed pairs (items 1; the bucket walk when there is no current list).
 template <class P>
-hipError_t run_sweep(fe_ctx* c, const char* name, int tid, const P& pol, unsigned long long* partial,
-                     const unsigned long long* prev, unsigned long long thr, uint32_t kind,
+hipError_t run_sweep(fe_ctx* c, const char* name, int tid, const P& pol, const unsigned long long* prev, unsigned long long thr, uint32_t kind,
                      unsigned long long* slot_out) {
-    if (!c->list_valid || c->tune[tid].items != 2) return run_list(c, name, tid, pol, partial, prev, thr, kind, slot_out);
+    if (!c->list_valid || c->tune[tid].items != 2) return run_list(c, name, tid, pol, prev, thr, kind, slot_out);
     if (c->win_gen != c->list_gen) {   // the current list has no window tables yet
         Launch W(c, "window_tables");
         constexpr int kTabWaves = 4;
@@ -1223,16 +1192,13 @@ hipError_t run_sweep(fe_ctx* c, const char* name, int tid, const P& pol, unsigne
     Launch L(c, name);
     const uint32_t block = c->tune[tid].block;
     if (c->nscenes > 1)
-        launch_win_b<P, true>(c, block, pol, partial, prev, thr, kind, slot_out);
+        launch_win_b<P, true>(c, block, pol, prev, thr, kind, slot_out);
     else
-        launch_win_b<P, false>(c, block, pol, partial, prev, thr, kind, slot_out);
+        launch_win_b<P, false>(c, block, pol, prev, thr, kind, slot_out);
     return hipGetLastError();
 }
 
 }  // namespace
-
-// Residual partials: one per block of the smallest gather block size.
-uint32_t gather_partials(uint32_t n) { return (n + 63) / 64; }
 
 hipError_t launch_schedule_mirrors(fe_ctx* c) {
     Launch L(c, "schedule_mirrors");
@@ -1334,7 +1300,7 @@ hipError_t launch_pressure_force(fe_ctx* c, const unsigned long long* prev, unsi
     hipError_t e = need(c, kMirrorPos | kMirrorPrho);
     if (e != hipSuccess) return e;
     PressureForce pol{c->pacc, c->pa_m, c->pa2_m};
-    e = run_sweep(c, "pressure_force", kTunePressureForce, pol, nullptr, prev, thr, kind, nullptr);
+    e = run_sweep(c, "pressure_force", kTunePressureForce, pol, prev, thr, kind, nullptr);
     c->mirrors |= kMirrorPacc;
     return e;
 }
@@ -1344,16 +1310,7 @@ hipError_t launch_pressure_update(fe_ctx* c, unsigned long long* out, const unsi
     hipError_t e = need(c, kMirrorPacc | kMirrorDens | kMirrorFac | kMirrorSrc | kMirrorPres);
     if (e != hipSuccess) return e;
     PressureUpdate pol{c->pa_m, c->pa2_m, c->fac_m, c->src_m, c->dens_m, c->pres, c->pres_m, c->pos_m};
-    if (c->nscenes > 1)   // per-scene sums straight into the scenes' slots
-        return run_sweep(c, "pressure_update", kTunePressureUpdate, pol, nullptr, prev, thr, kind, out);
-    e = run_sweep(c, "pressure_update", kTunePressureUpdate, pol, out ? c->partial : nullptr, prev, thr, kind,
-                  nullptr);
-    if (e != hipSuccess || !out) return e;
-    Launch L(c, "resid_finalize");
-    const uint32_t pb = c->tune[kTunePressureUpdate].block;
-    hipLaunchKernelGGL(k_resid_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->partial, (c->n + pb - 1) / pb, out,
-                       prev, thr);
-    return hipGetLastError();
+    return run_sweep(c, "pressure_update", kTunePressureUpdate, pol, prev, thr, kind, out);
 }
 
 }  // namespace fe

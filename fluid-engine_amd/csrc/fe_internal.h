@@ -224,7 +224,6 @@ struct fe_ctx {
     // stats_buffer and the convergence residuals (one slot per sweep, two solves)
     uint32_t* stats = nullptr;
     unsigned long long* resid = nullptr;   // 2 * kMaxSolveIters
-    unsigned long long* partial = nullptr; // per-wave residual partials of one sweep
     uint32_t* stats_partial = nullptr;     // 3 x kStatsBlocks per-block stats partials
     uint32_t* host_stats = nullptr;         // pinned [3]
     unsigned long long* host_resid = nullptr;  // pinned 2 * kMaxSolveIters
@@ -272,9 +271,7 @@ hipError_t sort_scratch_alloc(SortScratch* s, uint32_t max_n);
 void sort_scratch_free(SortScratch* s);
 
 // ---- launchers (fe_gather.hip) ----
-// Number of per-block residual partials the pressure_update gather writes for n particles.
-uint32_t gather_partials(uint32_t n);
-// Inverse schedule and the position / velocity_b mirrors after the schedule sort.
+// Inverse schedule and the position / velocity_b / velocity_a mirrors after the schedule sort.
 hipError_t launch_schedule_mirrors(fe_ctx* c);
 // velocity_b = velocity_a gathered through the current sort (reorder.comp's velocity half).
 hipError_t launch_reorder_velocity(fe_ctx* c);
@@ -284,6 +281,9 @@ hipError_t launch_reorder_velocity(fe_ctx* c);
 // constant-divisor quotient is checked on the device once per smoothing radius (cached per
 // process), the operand box of the gradient division from h on the host.
 hipError_t exact_math_flags(fe_ctx* c, const fe_params& p, uint32_t* flags);
+// fe_check_exact_arith on stream st with a 3-counter device scratch.
+hipError_t check_exact_arith(hipStream_t st, float h, unsigned long long samples, unsigned long long* dev_bad,
+                             unsigned long long counts[6]);
 hipError_t launch_hash(fe_ctx* c);
 hipError_t launch_offsets_reorder(fe_ctx* c);
 // Morton keys of the cells of position_b and the identity schedule payload.

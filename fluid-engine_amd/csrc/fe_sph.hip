@@ -449,6 +449,82 @@ std::unordered_map<uint32_t, bool> g_div_h_ok;   // by the bits of h
 
 }  // namespace
 
+namespace {
+
+// Mismatch counts of the exact shortcuts against the IEEE operations (fe_check_exact_arith).
+__global__ __launch_bounds__(kBlock) void k_check_sqrt(uint32_t lo, uint32_t count,
+                                                       unsigned long long* __restrict__ bad) {
+    const uint32_t stride = gridDim.x * kBlock;
+    unsigned long long miss = 0;
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count; k += stride) {
+        const float x = __uint_as_float(lo + k);
+        miss += __float_as_uint(sqrt_corrected(x)) != __float_as_uint(sqrtf(x));
+    }
+    if (miss) atomicAdd(bad, miss);
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_div(unsigned long long count, unsigned long long* __restrict__ bad) {
+    const unsigned long long stride = (unsigned long long)gridDim.x * kBlock;
+    unsigned long long miss = 0;
+    for (unsigned long long k = blockIdx.x * kBlock + threadIdx.x; k < count; k += stride) {
+        // splitmix64 of the sample index: signs, exponents in [-40, 39], mantissas
+        unsigned long long z = (k + 1ull) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const uint32_t a = (uint32_t)z, b = (uint32_t)(z >> 32);
+        const uint32_t ea = 87u + (a >> 23) % 80u, eb = 87u + (b >> 23) % 80u;   // biased 87..166
+        const float n = __uint_as_float(((a & 1u) << 31) | (ea << 23) | (a & 0x7FFFFFu));
+        const float d = __uint_as_float(((b & 1u) << 31) | (eb << 23) | (b & 0x7FFFFFu));
+        miss += __float_as_uint(div_boxed(n, d)) != __float_as_uint(n / d);
+    }
+    if (miss) atomicAdd(bad, miss);
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_div_h_count(float h, float inv_h, uint32_t lo, uint32_t count,
+                                                              unsigned long long* __restrict__ bad) {
+    const uint32_t stride = gridDim.x * kBlock;
+    unsigned long long miss = 0;
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count; k += stride) {
+        const float r = __uint_as_float(lo + k);
+        const float q0 = r * inv_h;
+        const float e = fmaf(-q0, h, r);
+        miss += __float_as_uint(fmaf(e, inv_h, q0)) != __float_as_uint(r / h);
+    }
+    if (miss) atomicAdd(bad, miss);
+}
+
+}  // namespace
+
+hipError_t check_exact_arith(hipStream_t st, float h, unsigned long long samples, unsigned long long* dev_bad,
+                             unsigned long long counts[6]) {
+    const float sq_lo_f = 0x1p-96f, sq_hi_f = 0x1p16f, r_lo_f = 1e-6f, r_hi_f = h * 1.001f;
+    uint32_t sq_lo, sq_hi, r_lo, r_hi;
+    std::memcpy(&sq_lo, &sq_lo_f, 4);
+    std::memcpy(&sq_hi, &sq_hi_f, 4);
+    std::memcpy(&r_lo, &r_lo_f, 4);
+    std::memcpy(&r_hi, &r_hi_f, 4);
+    counts[0] = sq_hi - sq_lo;
+    counts[2] = samples;
+    counts[4] = (h > 1e-6f && std::isfinite(r_hi_f)) ? (unsigned long long)(r_hi - r_lo + 1u) : 0ull;
+    hipError_t e = hipMemsetAsync(dev_bad, 0, 3 * sizeof(unsigned long long), st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_check_sqrt, dim3(4096), dim3(kBlock), 0, st, sq_lo, (uint32_t)counts[0], dev_bad);
+    if (samples) hipLaunchKernelGGL(k_check_div, dim3(4096), dim3(kBlock), 0, st, samples, dev_bad + 1);
+    if (counts[4])
+        hipLaunchKernelGGL(k_check_div_h_count, dim3(4096), dim3(kBlock), 0, st, h, 1.0f / h, r_lo,
+                           (uint32_t)counts[4], dev_bad + 2);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    unsigned long long bad[3];
+    e = hipMemcpyAsync(bad, dev_bad, sizeof(bad), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    counts[1] = bad[0];
+    counts[3] = bad[1];
+    counts[5] = bad[2];
+    return hipSuccess;
+}
+
 hipError_t exact_math_flags(fe_ctx* c, const fe_params& p, uint32_t* flags) {
     const float h = p.smoothing_radius;
     uint32_t hb;

@@ -184,6 +184,26 @@ fe_status fe_tune_set(fe_ctx* c, const char* kernel, uint32_t block, uint32_t it
     return fail(FE_ERR_INVALID_ARGUMENT, std::string("no tunable kernel named ") + kernel);
 }
 
+fe_status fe_check_exact_arith(int device, float h, uint64_t samples, uint64_t counts[6]) {
+    if (!counts) return fail(FE_ERR_INVALID_ARGUMENT, "null counts");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FE_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(FE_ERR_INVALID_ARGUMENT, "device index out of range");
+    if (!(h > 0.0f)) return fail(FE_ERR_INVALID_ARGUMENT, "h must be > 0");
+    DeviceGuard g(device);
+    hipStream_t st = nullptr;
+    unsigned long long* bad = nullptr;
+    FE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    hipError_t e = hipMalloc((void**)&bad, 3 * sizeof(unsigned long long));
+    unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
+    if (e == hipSuccess) e = fe::check_exact_arith(st, h, samples, bad, c);
+    if (bad) (void)hipFree(bad);
+    (void)hipStreamDestroy(st);
+    if (e != hipSuccess) return hip_fail(e, "exact arithmetic check");
+    for (int k = 0; k < 6; k++) counts[k] = c[k];
+    return FE_OK;
+}
+
 fe_status fe_device_name(const fe_ctx* c, char* name, size_t cap) {
     FE_CHECK_CTX(c);
     if (!name || cap == 0) return fail(FE_ERR_INVALID_ARGUMENT, "null name buffer");

@@ -172,6 +172,7 @@ _SIGNATURES = [
     ("fe_tune_get", [c_void_p, c_uint32, POINTER(FeTuneEntry)]),
     ("fe_tune_set", [c_void_p, c_char_p, c_uint32, c_uint32]),
     ("fe_device_name", [c_void_p, c_char_p, c_size_t]),
+    ("fe_check_exact_arith", [c_int, c_float, c_uint64, POINTER(c_uint64)]),
 ]
 
 EXPORTED_SYMBOLS = tuple(name for name, _ in _SIGNATURES)

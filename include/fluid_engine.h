@@ -215,6 +215,9 @@ fe_status fe_frame_control(const uint32_t stats[3], uint32_t n, const fe_params*
  * Tunable kernels: the six list gathers (density_alpha, viscosity, density_source,
  * divergence_source, pressure_force, pressure_update: block 64..1024), build_list (block
  * 64, 128, 256) and the two integrations (block 64..1024 x 1, 2 or 4 particles per thread).
+ * For the two Jacobi sweeps, items selects the variant: 1 gathers each listed pair from the
+ * schedule-ordered mirrors; 2 stages each 64-particle window's neighbour union in LDS and
+ * walks 16-bit lists (block 64, 128 or 256 = 1, 2 or 4 windows per block).
  * fe_autotune times every configuration on the context's current state (the median of
  * `reps` launches, 0 = 20) and keeps the fastest; the fields the timed launches overwrite
  * are restored, so the next frame is the one the context would have run. Configurations
@@ -233,6 +236,16 @@ fe_status fe_tune_get(const fe_ctx* ctx, uint32_t index, fe_tune_entry* out);
 fe_status fe_tune_set(fe_ctx* ctx, const char* kernel, uint32_t block, uint32_t items);
 /* "<gcn arch>/<compute units>cu" of the context's device (the tuning cache key). */
 fe_status fe_device_name(const fe_ctx* ctx, char* name, size_t cap);
+
+/* ---- exact arithmetic self-check (no reference counterpart) ----------------------------------
+ * The pair kernels compute square roots and quotients with shortcuts that equal the IEEE
+ * operations on the inputs they receive. This runs on `device` and compares each shortcut
+ * with the IEEE operation: the corrected square root for every float in [2^-96, 2^16); the
+ * boxed division for `samples` pseudo-random pairs with 2^-40 <= |n|, |d| < 2^40; r / h by
+ * the residual-corrected product for every float r in [1e-6, 1.001 h]. counts receives
+ * {sqrt checked, sqrt mismatches, division checked, division mismatches, r/h checked,
+ * r/h mismatches}. */
+fe_status fe_check_exact_arith(int device, float h, uint64_t samples, uint64_t counts[6]);
 
 /* ---- batched scenes (no reference counterpart) -----------------------------------------------
  * Many independent scenes of uneven sizes in one context, advanced together: every kernel of
