@@ -33,11 +33,14 @@ hipError_t launch_tuned(fe_ctx* c, int id) {
     }
 }
 
+// The staged-window sweeps (items 2) are not candidates: their window tables are built once
+// per list, a cost that timing sweep launches alone would leave out. fe_tune_set selects them.
 std::vector<fe::TuneCfg> candidates(int id) {
     std::vector<fe::TuneCfg> out;
+    const bool sweep = id == fe::kTunePressureForce || id == fe::kTunePressureUpdate;
     for (uint32_t b : {64u, 128u, 256u, 512u, 1024u})
         for (uint32_t it : {1u, 2u, 4u})
-            if (fe::tune_valid(id, b, it)) {
+            if (fe::tune_valid(id, b, it) && !(sweep && it == 2u)) {
                 fe::TuneCfg t;
                 t.block = b;
                 t.items = it;

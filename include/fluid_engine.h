@@ -217,7 +217,9 @@ fe_status fe_frame_control(const uint32_t stats[3], uint32_t n, const fe_params*
  * 64, 128, 256) and the two integrations (block 64..1024 x 1, 2 or 4 particles per thread).
  * For the two Jacobi sweeps, items selects the variant: 1 gathers each listed pair from the
  * schedule-ordered mirrors; 2 stages each 64-particle window's neighbour union in LDS and
- * walks 16-bit lists (block 64, 128 or 256 = 1, 2 or 4 windows per block).
+ * walks 16-bit lists (block 64, 128 or 256 = 1, 2 or 4 windows per block). fe_autotune
+ * tunes variant 1 only (variant 2 builds window tables once per list, which timing sweep
+ * launches alone would not count); fe_tune_set selects variant 2.
  * fe_autotune times every configuration on the context's current state (the median of
  * `reps` launches, 0 = 20) and keeps the fastest; the fields the timed launches overwrite
  * are restored, so the next frame is the one the context would have run. Configurations
