@@ -180,12 +180,20 @@ void frame_control(const uint32_t stats[3], uint32_t n, fe_params* p, const fe_o
 fe_status jacobi(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t* iterations);
 
 // Jacobi loop (src/renderer/mod.rs:301-306 / :327-332) between the warm-start hooks.
-// kind 0 = density, 1 = divergence.
+// kind 0 = density, 1 = divergence. The sweeps keep the new pressures and pressure
+// accelerations in their schedule-ordered mirrors; the hash-ordered fields are written once
+// when the loop ends.
 fe_status solve(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t* iterations) {
     FE_CHECK_CTX(c);
     fe_status s = ensure_ends(c);
     if (s == FE_OK) s = fe::learn_before_solve(c, kind);
-    if (s == FE_OK) s = jacobi(c, kind, fixed_iters, tol, iterations);
+    if (s == FE_OK) {
+        c->defer_sweep_writes = true;
+        s = jacobi(c, kind, fixed_iters, tol, iterations);
+        c->defer_sweep_writes = false;
+        hipError_t e = fe::flush_sweep_writes(c);
+        if (s == FE_OK && e != hipSuccess) s = hip_fail(e, "sweep write-back");
+    }
     if (s == FE_OK) s = fe::learn_after_solve(c, kind);
     return s;
 }

@@ -820,7 +820,7 @@ struct PressureForce {
     }
     __device__ __forceinline__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si, const Sph& s) const {
         const float4 v = make_float4(-a.x, -a.y, -a.z, 0.0f);
-        pacc[i] = v;
+        if (pacc) pacc[i] = v;   // null inside a solve: written back afterwards
         pa_m[t] = make_float4(si.px, si.py, si.pz, v.x);
         pa2_m[t] = make_float2(v.y, v.z);
         return 0;
@@ -883,7 +883,7 @@ struct PressureUpdate {
                 const float corr = (s.omega * err) / a_ii;
                 const float np = p + corr;
                 p = np > 0.0f ? np : 0.0f;
-                pres[i] = p;
+                if (pres) pres[i] = p;   // null inside a solve: written back afterwards
                 pres_m[t] = p;
             }
         }
@@ -1109,9 +1109,29 @@ __global__ __launch_bounds__(64 * kWaves) void k_sweep_win(P pol, Sph s0, Scenes
 
 inline dim3 grid_for(uint32_t count) { return dim3((count + kBlock - 1) / kBlock); }
 
-// Mirror bits each consumer needs; a missing one is rebuilt from the hash-ordered fields.
+// The hash-ordered pressures and pressure accelerations of the sweeps that deferred them.
+__global__ __launch_bounds__(kBlock) void k_sweep_writeback(const uint32_t* __restrict__ sched, uint32_t n,
+                                                            const float* __restrict__ pres_m,
+                                                            const float4* __restrict__ pa_m,
+                                                            const float2* __restrict__ pa2_m, float* __restrict__ pres,
+                                                            float4* __restrict__ pacc) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = sched[t];
+    if (pres) pres[i] = pres_m[t];
+    if (pacc) {
+        const float ax = pa_m[t].w;
+        const float2 b = pa2_m[t];
+        pacc[i] = make_float4(ax, b.x, b.y, 0.0f);
+    }
+}
+
+// Mirror bits each consumer needs; a missing one is rebuilt from the hash-ordered fields
+// (after any deferred sweep writes have reached them).
 hipError_t need(fe_ctx* c, uint32_t bits) {
     if ((c->mirrors & bits) == bits) return hipSuccess;
+    hipError_t e = flush_sweep_writes(c);
+    if (e != hipSuccess) return e;
     Launch L(c, "refresh_mirrors");
     hipLaunchKernelGGL(k_refresh_mirrors, grid_for(c->n), dim3(kBlock), 0, c->stream, c->sched[0], c->n, c->pos_b,
                        c->vel_b, c->vel_a, c->dens, c->pres, c->pacc, c->fac, c->src, c->inv, c->pos_m, c->vb_m,
@@ -1299,9 +1319,10 @@ hipError_t launch_divergence_source(fe_ctx* c) {
 hipError_t launch_pressure_force(fe_ctx* c, const unsigned long long* prev, unsigned long long thr, uint32_t kind) {
     hipError_t e = need(c, kMirrorPos | kMirrorPrho);
     if (e != hipSuccess) return e;
-    PressureForce pol{c->pacc, c->pa_m, c->pa2_m};
+    PressureForce pol{c->defer_sweep_writes ? nullptr : c->pacc, c->pa_m, c->pa2_m};
     e = run_sweep(c, "pressure_force", kTunePressureForce, pol, prev, thr, kind, nullptr);
     c->mirrors |= kMirrorPacc;
+    c->pending_pacc |= c->defer_sweep_writes;
     return e;
 }
 
@@ -1309,8 +1330,20 @@ hipError_t launch_pressure_update(fe_ctx* c, unsigned long long* out, const unsi
                                   unsigned long long thr, uint32_t kind) {
     hipError_t e = need(c, kMirrorPacc | kMirrorDens | kMirrorFac | kMirrorSrc | kMirrorPres);
     if (e != hipSuccess) return e;
-    PressureUpdate pol{c->pa_m, c->pa2_m, c->fac_m, c->src_m, c->dens_m, c->pres, c->pres_m, c->pos_m};
-    return run_sweep(c, "pressure_update", kTunePressureUpdate, pol, prev, thr, kind, out);
+    PressureUpdate pol{c->pa_m,   c->pa2_m,  c->fac_m, c->src_m, c->dens_m, c->defer_sweep_writes ? nullptr : c->pres,
+                       c->pres_m, c->pos_m};
+    e = run_sweep(c, "pressure_update", kTunePressureUpdate, pol, prev, thr, kind, out);
+    c->pending_pres |= c->defer_sweep_writes;
+    return e;
+}
+
+hipError_t flush_sweep_writes(fe_ctx* c) {
+    if (!c->pending_pres && !c->pending_pacc) return hipSuccess;
+    Launch L(c, "sweep_writeback");
+    hipLaunchKernelGGL(k_sweep_writeback, grid_for(c->n), dim3(kBlock), 0, c->stream, c->sched[0], c->n, c->pres_m,
+                       c->pa_m, c->pa2_m, c->pending_pres ? c->pres : nullptr, c->pending_pacc ? c->pacc : nullptr);
+    c->pending_pres = c->pending_pacc = false;
+    return hipGetLastError();
 }
 
 }  // namespace fe

@@ -198,6 +198,10 @@ struct fe_ctx {
     float* src_m = nullptr;      // source_terms
     float* pres_m = nullptr;     // pressures
     uint32_t mirrors = 0;        // kMirror* bits that are current
+    // Inside a solve the sweeps leave the hash-ordered pressures / pressure accelerations to
+    // their mirrors; one write-back after the solve (or before any mirror refresh) stores them.
+    bool defer_sweep_writes = false;
+    bool pending_pres = false, pending_pacc = false;
     // sweep windows of the current list (valid with it): per window the staged entry count K
     // and run count R (K | R << 16, or kWinFallback), per staging round the 64-bit mask of
     // runs that start inside the round and the run holding its first entry, per run the
@@ -303,6 +307,9 @@ hipError_t launch_density_source(fe_ctx* c);
 hipError_t launch_pressure_force(fe_ctx* c, const unsigned long long* prev, unsigned long long thr, uint32_t kind);
 hipError_t launch_pressure_update(fe_ctx* c, unsigned long long* out, const unsigned long long* prev,
                                   unsigned long long thr, uint32_t kind);
+// The hash-ordered pressures / pressure accelerations from their mirrors after sweeps that
+// deferred them (no-op when nothing is pending).
+hipError_t flush_sweep_writes(fe_ctx* c);
 hipError_t launch_pressure_integration(fe_ctx* c);
 hipError_t launch_divergence_source(fe_ctx* c);
 hipError_t launch_divergence_integration(fe_ctx* c);
