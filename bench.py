@@ -8,8 +8,9 @@ divergence integration, stats; at most 100 sweeps per solve). Inputs are residen
 before the timed region; the region is bracketed by HIP events on the context's stream.
 
 Prints one JSON line: steps/s (value), particle-updates/s, Jacobi sweeps per step, the
-roofline of the dominant kernel (algorithmic bytes / event-timed average launch) and a
-CPU baseline (the oracle on the host cores, bounded sample).
+roofline of the dominant kernel (algorithmic bytes / event-timed average launch, and its
+VALU issue rate: the gathers are bound by vector-ALU issue, not by HBM) and a CPU baseline
+(the oracle on the host cores, bounded sample).
 """
 import argparse
 import json
@@ -96,6 +97,24 @@ def traffic_for(kernel):
     with open(path) as f:
         t = json.load(f).get("kernels", {}).get(kernel)
     return None if t is None else t.get("hbm_bytes_per_launch")
+
+
+def valu_for(kernel, avg_launch_us):
+    """VALU issue roofline of `kernel`: its wave64 VALU instructions per launch (from the
+    committed rocprofv3 SQ counter summary) over the event-timed average launch, against the
+    chip's issue rate (one wave64 VALU instruction per SIMD per 4 cycles)."""
+    path = os.path.join(ROOT, "profiles", "valu_dam_break_1m.json")
+    if not os.path.exists(path) or not avg_launch_us:
+        return None
+    with open(path) as f:
+        doc = json.load(f)
+    k = doc.get("kernels", {}).get(kernel)
+    if k is None:
+        return None
+    achieved = k["valu_per_launch"] / (avg_launch_us * 1e3)   # G wave-instructions/s
+    peak = doc["issue_peak_ginstr_per_s"]
+    return {"achieved": round(achieved, 1), "peak": peak, "unit": "G wave64 VALU instr/s",
+            "frac": round(achieved / peak, 4), "valu_per_launch": k["valu_per_launch"]}
 
 
 def main():
@@ -213,7 +232,8 @@ def main():
         "profile_sweeps": {"density": d_avg, "divergence": v_avg},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(dom),
-                     "algo_bytes_per_particle": dom_bytes, "avg_launch_us": kernels[dom]["avg_us"]},
+                     "algo_bytes_per_particle": dom_bytes, "avg_launch_us": kernels[dom]["avg_us"],
+                     "valu_issue": valu_for(dom, kernels[dom]["avg_us"])},
         "cpu_baseline": cpu,
         "autotune": None if tuned is None else {
             "untuned_steps_per_s": untuned,
