@@ -610,6 +610,7 @@ fe_status fe_set_params(fe_ctx* c, const fe_params* p) {
     c->params = *p;
     c->exact_fast = fast;
     c->morton_bits = morton_bits_for(*p);
+    c->mirrors &= ~fe::kMirrorMrho;   // holds the old mass
     return FE_OK;
 }
 
@@ -679,7 +680,7 @@ fe_status fe_upload(fe_ctx* c, int field, const void* src, size_t bytes) {
         case FE_FIELD_POSITION_B: c->mirrors &= ~(fe::kMirrorPos | fe::kMirrorPacc); break;
         case FE_FIELD_VELOCITY_B: c->mirrors &= ~fe::kMirrorVelB; break;
         case FE_FIELD_VELOCITY_A: c->mirrors &= ~fe::kMirrorVelA; break;
-        case FE_FIELD_DENSITIES: c->mirrors &= ~(fe::kMirrorDens | fe::kMirrorPrho); break;
+        case FE_FIELD_DENSITIES: c->mirrors &= ~(fe::kMirrorDens | fe::kMirrorPrho | fe::kMirrorMrho); break;
         case FE_FIELD_FACTORS: c->mirrors &= ~fe::kMirrorFac; break;
         case FE_FIELD_SOURCE_TERMS: c->mirrors &= ~fe::kMirrorSrc; break;
         case FE_FIELD_PRESSURES: c->mirrors &= ~(fe::kMirrorPrho | fe::kMirrorPres); break;
@@ -1059,6 +1060,7 @@ fe_status fe_batch_set_params(fe_ctx* c, uint32_t scene, const fe_params* p) {
     drop_graph(c);
     c->scene_params[scene] = *p;
     c->ns_fresh = c->da_fresh = false;   // h may have changed
+    c->mirrors &= ~fe::kMirrorMrho;      // holds the old mass
     return upload_scene_tab(c);
 }
 

@@ -184,10 +184,15 @@ __device__ __forceinline__ float q_of(float r, const Sph& s) {
     return r / s.h;
 }
 
-// common.glsl:38-54 kernel_w with IEEE q = r / h (r <= 1.001 h); both polynomial pieces are
-// evaluated and one is selected.
-__device__ __forceinline__ float exact_w(float r, const Sph& s) {
-    const float q = q_of(r, s);
+// q = r / h (IEEE) for 1e-6 <= r <= 1.001 h, without the range test; for smaller r the
+// result may differ, so only terms that skip r <= 1e-6 may use it.
+__device__ __forceinline__ float q_big(float r, const Sph& s) {
+    return (s.fast & kFastDivH) ? div_by_h(r, s) : r / s.h;
+}
+
+// common.glsl:38-54 kernel_w of q = r / h; both polynomial pieces are evaluated and one is
+// selected.
+__device__ __forceinline__ float w_of_q(float q, const Sph& s) {
     const float q2 = q * q;
     const float q3 = q2 * q;
     const float inner = s.k * (6.0f * q3 - 6.0f * q2 + 1.0f);
@@ -197,12 +202,11 @@ __device__ __forceinline__ float exact_w(float r, const Sph& s) {
     const float w = q <= 0.5f ? inner : outer;
     return q >= 1.0f ? 0.0f : w;
 }
+__device__ __forceinline__ float exact_w(float r, const Sph& s) { return w_of_q(q_of(r, s), s); }
 
-// common.glsl:56-73 kernel_grad; the scalar that multiplies r_vec (r <= 1.001 h). It is 0 for
-// r < 1e-6, so q needs to be exact only from 1e-6 on: the residual-corrected product serves
-// without a range test.
-__device__ __forceinline__ float exact_grad(float r, const Sph& s) {
-    const float q = (s.fast & kFastDivH) ? div_by_h(r, s) : r / s.h;
+// common.glsl:56-73 kernel_grad of r and q = r / h; the scalar that multiplies r_vec
+// (r <= 1.001 h). It is 0 for r < 1e-6, so q needs to be exact only from 1e-6 on.
+__device__ __forceinline__ float grad_of_q(float r, float q, const Sph& s) {
     const float inner = s.k * (18.0f * q * q - 12.0f * q);
     const float omq = 1.0f - q;
     const float outer = s.k * -6.0f * omq * omq;
@@ -215,6 +219,7 @@ __device__ __forceinline__ float exact_grad(float r, const Sph& s) {
         g = gf / d;
     return (q >= 1.0f || r < 1e-6f) ? 0.0f : g;
 }
+__device__ __forceinline__ float exact_grad(float r, const Sph& s) { return grad_of_q(r, q_big(r, s), s); }
 
 // p / rho^2 of a particle whose density is <= 1e-6: pressure_force skips such neighbours.
 // A signalling-NaN payload that no arithmetic result carries.

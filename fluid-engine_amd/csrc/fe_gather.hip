@@ -17,8 +17,8 @@
 // is therefore scheduled in Morton order of the particles' cells (work-item t handles
 // particle sched[t]) and the data a gather reads from its neighbours is kept in
 // schedule-ordered mirrors that the producing stage writes next to its hash-ordered output:
-// pos_m (position_b, with p / rho^2 in w), vb_m, va_m, dens_m, and pa_m / pa2_m (position
-// and pressure acceleration packed for pressure_update). The list holds schedule
+// pos_m (position_b, with p / rho^2 in w), vb_m (velocity_b, with m / rho in w), va_m,
+// dens_m, and pa_m / pa2_m (position and pressure acceleration packed for pressure_update). The list holds schedule
 // positions in tiles of 16 work-items x 4 slots, so 16 lanes reading four entries each
 // read 256 contiguous bytes, a cell group's entries fill whole lines while the group is
 // built, and each lane keeps four independent neighbour loads in flight.
@@ -93,7 +93,8 @@ __global__ __launch_bounds__(kBlock) void k_schedule_mirrors(const uint32_t* __r
     va_m[t] = vel_a[i];
 }
 
-// reorder.comp:44 for the velocities alone, written to velocity_b and its mirror.
+// reorder.comp:44 for the velocities alone, written to velocity_b and the xyz of its mirror
+// (whose w keeps the m / rho of the unchanged densities).
 __global__ __launch_bounds__(kBlock) void k_reorder_velocity(const uint32_t* __restrict__ sched,
                                                              const uint32_t* __restrict__ vals, uint32_t n,
                                                              const float4* __restrict__ vel_a,
@@ -103,11 +104,15 @@ __global__ __launch_bounds__(kBlock) void k_reorder_velocity(const uint32_t* __r
     const uint32_t i = sched[t];
     const float4 v = vel_a[vals[i]];
     vel_b[i] = v;
-    vb_m[t] = v;
+    float* m = reinterpret_cast<float*>(vb_m + t);
+    m[0] = v.x;
+    m[1] = v.y;
+    m[2] = v.z;
 }
 
 // Every mirror from the hash-ordered fields (after uploads or out-of-sequence calls).
-__global__ __launch_bounds__(kBlock) void k_refresh_mirrors(const uint32_t* __restrict__ sched, uint32_t n,
+__global__ __launch_bounds__(kBlock) void k_refresh_mirrors(Sph s0, Scenes sc, const uint32_t* __restrict__ sched,
+                                                            uint32_t n,
                                                             const float4* __restrict__ pos_b,
                                                             const float4* __restrict__ vel_b,
                                                             const float4* __restrict__ vel_a,
@@ -127,7 +132,8 @@ __global__ __launch_bounds__(kBlock) void k_refresh_mirrors(const uint32_t* __re
     const float4 x = pos_b[i];
     const float rho = dens[i], p = pres[i];
     pos_m[t] = make_float4(x.x, x.y, x.z, prho_of(p, rho));
-    vb_m[t] = vel_b[i];
+    const float4 vb = vel_b[i];
+    vb_m[t] = make_float4(vb.x, vb.y, vb.z, scene_at(sc, s0, i).mass / rho);
     va_m[t] = vel_a[i];
     dens_m[t] = rho;
     const float4 a = pacc[i];
@@ -514,6 +520,7 @@ struct DensityAlpha {
     float* fac;
     float* dens_m;
     float* fac_m;
+    float4* vb_w;   // the velocity_b mirror, whose w takes m / rho
     struct Self {};
     struct Cand {};
     struct Acc { float rho, sgs, gx, gy, gz; };
@@ -526,10 +533,11 @@ struct DensityAlpha {
         const float r2 = rx * rx + ry * ry + rz * rz;
         const bool in = !(r2 > s.hh);
         const float r = sqrt_exact(r2);
-        const float w = exact_w(r, s);
+        const float q = q_of(r, s);
+        const float w = w_of_q(q, s);
         a.rho += in ? s.mass * w : 0.0f;
         const bool gok = in && r > 1e-6f;
-        const float g = exact_grad(r, s);
+        const float g = grad_of_q(r, q, s);
         const float mx = s.mass * (g * rx), my = s.mass * (g * ry), mz = s.mass * (g * rz);
         a.gx += gok ? mx : 0.0f;
         a.gy += gok ? my : 0.0f;
@@ -539,6 +547,7 @@ struct DensityAlpha {
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         dens[i] = a.rho;
         dens_m[t] = a.rho;
+        vb_w[t].w = s.mass / a.rho;
         const float alpha = a.sgs + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
         const float f = alpha > 1e-6f ? alpha : 0.0f;
         fac[i] = f;
@@ -547,26 +556,30 @@ struct DensityAlpha {
     }
 };
 
-// viscosity.comp:17-66 (velocity_b in, velocity_a out with w = 0, gravity fused)
+// viscosity.comp:17-66 (velocity_b in, velocity_a out with w = 0, gravity fused). The
+// neighbour's m / rho_j comes in the w of its velocity_b mirror (written with its density,
+// the same IEEE quotient the shader forms per pair); W is needed only where r > 1e-6.
 struct Viscosity {
     const float4* vb_m;
-    const float* dens_m;
     float4* vel_a;
     float4* va_m;
     struct Self { float4 v; };
-    struct Cand { float4 v; float rho; };
+    struct Cand { float4 v; float mrho; };
     struct Acc { float x, y, z; };
     static constexpr bool kPaGeo = false;
     __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {vb_m[t]}; }
-    __device__ Cand load(uint32_t jt, const float4&) const { return {vb_m[jt], dens_m[jt]}; }
+    __device__ Cand load(uint32_t jt, const float4&) const {
+        const float4 v = vb_m[jt];
+        return {v, v.w};
+    }
     __device__ Acc zero() const { return {0.f, 0.f, 0.f}; }
     __device__ __forceinline__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
                                          const Cand& c) const {
         const float r2 = rx * rx + ry * ry + rz * rz;
-        const float r = sqrt_exact(r2);
+        const float r = sqrt_grad(r2);
         const bool ok = !(r2 > s.hh) && r > 1e-6f;
-        const float w = exact_w(r, s);
-        const float m_rho = s.mass / c.rho;
+        const float w = w_of_q(q_big(r, s), s);
+        const float m_rho = c.mrho;
         a.x += ok ? (m_rho * (c.v.x - si.v.x)) * w : 0.0f;
         a.y += ok ? (m_rho * (c.v.y - si.v.y)) * w : 0.0f;
         a.z += ok ? (m_rho * (c.v.z - si.v.z)) * w : 0.0f;
@@ -1133,9 +1146,10 @@ hipError_t need(fe_ctx* c, uint32_t bits) {
     hipError_t e = flush_sweep_writes(c);
     if (e != hipSuccess) return e;
     Launch L(c, "refresh_mirrors");
-    hipLaunchKernelGGL(k_refresh_mirrors, grid_for(c->n), dim3(kBlock), 0, c->stream, c->sched[0], c->n, c->pos_b,
-                       c->vel_b, c->vel_a, c->dens, c->pres, c->pacc, c->fac, c->src, c->inv, c->pos_m, c->vb_m,
-                       c->va_m, c->dens_m, c->pa_m, c->pa2_m, c->fac_m, c->src_m, c->pres_m);
+    hipLaunchKernelGGL(k_refresh_mirrors, grid_for(c->n), dim3(kBlock), 0, c->stream, sph_of(c), scenes_of(c),
+                       c->sched[0], c->n, c->pos_b, c->vel_b, c->vel_a, c->dens, c->pres, c->pacc, c->fac, c->src,
+                       c->inv, c->pos_m, c->vb_m, c->va_m, c->dens_m, c->pa_m, c->pa2_m, c->fac_m,
+                       c->src_m, c->pres_m);
     c->mirrors = kMirrorAll;
     return hipGetLastError();
 }
@@ -1263,9 +1277,9 @@ hipError_t launch_build_list(fe_ctx* c) {
 }
 
 hipError_t launch_density_alpha_only(fe_ctx* c) {
-    DensityAlpha pol{c->dens, c->fac, c->dens_m, c->fac_m};
+    DensityAlpha pol{c->dens, c->fac, c->dens_m, c->fac_m, c->vb_m};
     hipError_t e = run_list(c, "density_alpha", kTuneDensityAlpha, pol);
-    c->mirrors = (c->mirrors | kMirrorDens | kMirrorFac) & ~kMirrorPrho;   // p / rho^2 used the old densities
+    c->mirrors = (c->mirrors | kMirrorDens | kMirrorFac | kMirrorMrho) & ~kMirrorPrho;   // p / rho^2 used the old densities
     c->da_fresh = c->ns_fresh;
     return e;
 }
@@ -1276,9 +1290,9 @@ hipError_t launch_density_alpha(fe_ctx* c) {
 }
 
 hipError_t launch_viscosity(fe_ctx* c) {
-    hipError_t e = need(c, kMirrorPos | kMirrorVelB | kMirrorDens);
+    hipError_t e = need(c, kMirrorPos | kMirrorVelB | kMirrorMrho);
     if (e != hipSuccess) return e;
-    Viscosity pol{c->vb_m, c->dens_m, c->vel_a, c->va_m};
+    Viscosity pol{c->vb_m, c->vel_a, c->va_m};
     e = run_list(c, "viscosity", kTuneViscosity, pol);
     c->mirrors |= kMirrorVelA;
     return e;

@@ -21,7 +21,7 @@ constexpr uint32_t kMaxNbr = 64;          // neighbour-list slots per particle
 // per-particle solver fields the Jacobi sweeps read for their own particle
 constexpr uint32_t kMirrorPos = 1u, kMirrorVelB = 2u, kMirrorVelA = 4u, kMirrorDens = 8u, kMirrorPrho = 16u,
                    kMirrorPacc = 32u, kMirrorFac = 64u, kMirrorSrc = 128u, kMirrorPres = 256u,
-                   kMirrorAll = 511u;
+                   kMirrorMrho = 512u, kMirrorAll = 1023u;
 // Windows of the Jacobi sweeps: 64 consecutive schedule positions (one wave). The union of
 // the window's listed neighbours is staged in LDS once per sweep, as at most kWinMaxRuns
 // runs of consecutive schedule positions holding at most kWinMaxK entries in total.
@@ -186,7 +186,7 @@ struct fe_ctx {
     uint32_t scene_bits = 0;     // bits of the scene number above them (a batch)
     uint32_t* inv = nullptr;     // inverse schedule: hash index -> schedule position
     float4* pos_m = nullptr;     // position_b by schedule position
-    float4* vb_m = nullptr;      // velocity_b
+    float4* vb_m = nullptr;      // velocity_b, with mass / density in w (kMirrorMrho)
     float4* va_m = nullptr;      // velocity_a
     float* dens_m = nullptr;     // densities
     // pos_m.w holds pressure / density^2 (kMirrorPrho); pa_m = (position_b xyz, pressure
