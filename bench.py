@@ -67,6 +67,7 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="run the default launch configurations")
     ap.add_argument("--tune-cache", default=None, help="JSON cache of tuned configurations (read and written)")
     ap.add_argument("--batch-frames", type=int, default=10)
+    ap.add_argument("--no-sort-bench", action="store_true", help="skip the isolated 2^22 sort measurement")
     ap.add_argument("--learned-collect", type=int, default=100, help="frames sampled before training")
     ap.add_argument("--learned-eval", type=int, default=400, help="frames compared after training")
     ap.add_argument("--copy-probe", action="store_true",
@@ -245,7 +246,44 @@ def main():
         out["learned_warm_start"] = learned_comparison(fe, w, opts, args.learned_collect, args.learned_eval, tuned)
     if not args.no_batch:
         out["batched_scenes"] = batched_measurement(fe, scene, args.batch_frames)
+    if not args.no_sort_bench:
+        out["isolated_sort_2p22"] = sort_measurement(fe, scene)
     print(json.dumps(out), flush=True)
+
+
+def sort_measurement(fe, scene, log2n=22, warmup=3, reps=15):
+    """The reference's isolated sort benchmark (sorter.rs:869-924, scripts/sort_benchmark.csv:20-21):
+    2^22 {hash, index} pairs with random u32 keys, median of `reps` sorts after `warmup`, each
+    sort timed alone with HIP events on the context's stream (fresh random keys uploaded
+    before each one, outside the timed region). fe_sort sorts all entries on 32 bits (radix:
+    four 8-bit passes)."""
+    import numpy as np
+    n = 1 << log2n
+    rng = np.random.default_rng(22)
+    params = scene.make_params(0.01, 0.008, 0.04, 1000, 0.01, 0.5, 1e-3, 1, 1, (0, -9.81, 0), (0, 0, 0), (1, 1, 1))
+    out = {"pairs": n, "keys": "random u32", "protocol": f"median of {reps} after {warmup} warm-up, HIP events"}
+    for name, algo in (("radix", fe.FE_SORT_RADIX), ("bitonic", fe.FE_SORT_BITONIC)):
+        o = fe.default_options()
+        o.sort_algorithm = algo
+        sim = fe.Simulation(params, np.zeros((n, 3), np.float32), device=0, options=o)
+        times = []
+        for r in range(warmup + reps):
+            keys = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)
+            sim.upload("grid_entries", np.stack([keys, np.arange(n, dtype=np.uint32)], axis=1))
+            sim.sync()
+            sim.timer_start()
+            sim.sort()
+            ms = sim.timer_stop()
+            if r >= warmup:
+                times.append(ms)
+        sim.close()
+        times.sort()
+        out[f"{name}_ms"] = round(times[len(times) // 2], 4)
+    out["reference"] = {"radix_ms": 27.5775, "bitonic_ms": 82.766, "hardware": "unstated GPU (Windows, Vulkan)",
+                        "measures": "submit-to-fence wall clock", "source": "scripts/sort_benchmark.csv:20-21"}
+    # 68 algorithmic bytes per pair for a four-pass LSD sort of 8-byte pairs, at 8 TB/s
+    out["radix_hbm_floor_ms"] = round(68 * n / 8e12 * 1e3, 4)
+    return out
 
 
 def learned_comparison(fe, w, opts, collect, evaluate, tuned=None):
