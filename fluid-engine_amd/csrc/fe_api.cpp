@@ -80,6 +80,9 @@ void free_ctx(fe_ctx* c) {
     }
     for (hipEvent_t e : {c->stats_event, c->timer_start, c->timer_stop})
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->tab_event)
+        if (e) (void)hipEventDestroy(e);
+    if (c->tab_ring) (void)hipHostFree(c->tab_ring);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -124,9 +127,7 @@ fe_status ensure_staging(fe_ctx* c) {
 }
 
 // The scene table of a batch: per-scene constants and converge thresholds (tol x N_s x 2^32).
-fe_status upload_scene_tab(fe_ctx* c) {
-    if (c->nscenes <= 1) return FE_OK;
-    std::vector<fe::Sph> tab(c->nscenes);
+fe_status fill_scene_tab(fe_ctx* c, fe::Sph* tab) {
     for (uint32_t k = 0; k < c->nscenes; k++) {
         uint32_t fast = 0;
         FE_HIP(fe::exact_math_flags(c, c->scene_params[k], &fast), "exact arithmetic check");
@@ -135,8 +136,105 @@ fe_status upload_scene_tab(fe_ctx* c) {
         tab[k].thr[0] = (unsigned long long)((double)c->options.density_tol * (double)c->scene_n[k] * 4294967296.0);
         tab[k].thr[1] = (unsigned long long)((double)c->options.divergence_tol * (double)c->scene_n[k] * 4294967296.0);
     }
+    return FE_OK;
+}
+
+fe_status upload_scene_tab(fe_ctx* c) {
+    if (c->nscenes <= 1) return FE_OK;
+    std::vector<fe::Sph> tab(c->nscenes);
+    fe_status s = fill_scene_tab(c, tab.data());
+    if (s != FE_OK) return s;
     FE_HIP(hipStreamSynchronize(c->stream), "scene table wait");
     FE_HIP(hipMemcpy(c->scene_tab, tab.data(), sizeof(fe::Sph) * tab.size(), hipMemcpyHostToDevice), "scene table");
+    return FE_OK;
+}
+
+// The same copy ordered on the stream without waiting for it: the table is written into one
+// of kTabRing pinned host slots, whose previous copy (kTabRing frames ago) has long finished.
+fe_status enqueue_scene_tab(fe_ctx* c) {
+    const size_t per = sizeof(fe::Sph) * c->nscenes;
+    if (!c->tab_ring) {
+        FE_HIP(hipHostMalloc((void**)&c->tab_ring, per * fe::kTabRing, hipHostMallocDefault), "hipHostMalloc tables");
+        for (hipEvent_t& e : c->tab_event) FE_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), "table event");
+    }
+    const uint32_t slot = c->tab_slot;
+    c->tab_slot = (slot + 1) % fe::kTabRing;
+    FE_HIP(hipEventSynchronize(c->tab_event[slot]), "table slot wait");
+    fe::Sph* tab = c->tab_ring + (size_t)slot * c->nscenes;
+    fe_status s = fill_scene_tab(c, tab);
+    if (s != FE_OK) return s;
+    FE_HIP(hipMemcpyAsync(c->scene_tab, tab, per, hipMemcpyHostToDevice, c->stream), "scene table");
+    FE_HIP(hipEventRecord(c->tab_event[slot], c->stream), "table event");
+    return FE_OK;
+}
+
+// CollisionBox::update (src/entities/collision.rs:31-35). Each operation rounds to float in
+// the reference's order (no contraction: the library is built with -ffp-contract=off), and
+// the sine is the C library's sinf, which Rust's f32::sin calls. The constant is
+// std::f32::consts::TAU rounded to float.
+void wall_update(fe_wall* w, float dt) {
+    constexpr float kTau = 6.28318530717958647692f;
+    w->time = w->time + dt;
+    const float phase = w->time * w->wave_frequency * kTau;
+    const float offset = w->wave_amplitude * std::sin(phase);
+    w->min[0] = w->base_min_x + offset;
+}
+
+// The UI pass of the reference frame (src/renderer/ui.rs:125-126): the box enters the
+// parameter block with w = 0.
+void wall_to_params(const fe_wall& w, fe_params* p) {
+    for (int a = 0; a < 3; a++) {
+        p->box_min[a] = w.min[a];
+        p->box_max[a] = w.max[a];
+    }
+    p->box_min[3] = 0.0f;
+    p->box_max[3] = 0.0f;
+}
+
+// Before a frame: every attached wall's box goes into its scene's parameters.
+fe_status apply_walls(fe_ctx* c) {
+    if (!c->walls_on) return FE_OK;
+    if (c->nscenes == 1) {
+        wall_to_params(c->walls[0], &c->params);
+        c->morton_bits = morton_bits_for(c->params);
+        return FE_OK;
+    }
+    for (uint32_t k = 0; k < c->nscenes; k++)
+        if (c->wall_on[k]) wall_to_params(c->walls[k], &c->scene_params[k]);
+    return enqueue_scene_tab(c);
+}
+
+// After a frame (src/renderer/mod.rs:270): every attached wall advances by max_dt.
+void advance_walls(fe_ctx* c, float max_dt) {
+    for (uint32_t k = 0; k < c->walls.size(); k++)
+        if (c->wall_on[k]) wall_update(&c->walls[k], max_dt);
+}
+
+bool wall_finite(const fe_wall& w) {
+    const float* f = &w.min[0];
+    for (size_t k = 0; k < sizeof(fe_wall) / sizeof(float); k++)
+        if (!std::isfinite(f[k])) return false;
+    return true;
+}
+
+fe_status set_wall(fe_ctx* c, uint32_t scene, const fe_wall* w) {
+    if (w && !wall_finite(*w)) return fail(FE_ERR_INVALID_ARGUMENT, "wall fields must be finite");
+    if (c->walls.size() < c->nscenes) {
+        c->walls.resize(c->nscenes, fe_wall{});
+        c->wall_on.resize(c->nscenes, 0);
+    }
+    const bool was = c->wall_on[scene] != 0;
+    c->wall_on[scene] = w ? 1 : 0;
+    c->walls[scene] = w ? *w : fe_wall{};
+    c->walls_on = c->walls_on - (was ? 1 : 0) + (w ? 1 : 0);
+    return FE_OK;
+}
+
+fe_status get_wall(const fe_ctx* c, uint32_t scene, fe_wall* out, int* attached) {
+    if (!out) return fail(FE_ERR_INVALID_ARGUMENT, "null out");
+    const bool on = scene < c->wall_on.size() && c->wall_on[scene];
+    *out = on ? c->walls[scene] : fe_wall{};
+    if (attached) *attached = on ? 1 : 0;
     return FE_OK;
 }
 
@@ -422,8 +520,12 @@ fe_status step_graph(fe_ctx* c, float max_dt) {
 }
 
 fe_status run_frame(fe_ctx* c, float max_dt) {
+    fe_status s = apply_walls(c);
+    if (s != FE_OK) return s;
     const bool graph = c->batch_graph && c->nscenes > 1 && c->options.solver_mode == FE_SOLVER_FIXED && !c->profiling;
-    return graph ? step_graph(c, max_dt) : step(c, max_dt);
+    s = graph ? step_graph(c, max_dt) : step(c, max_dt);
+    if (s == FE_OK) advance_walls(c, max_dt);
+    return s;
 }
 
 }  // namespace
@@ -940,6 +1042,40 @@ fe_status fe_frame_control(const uint32_t stats[3], uint32_t n, const fe_params*
     return FE_OK;
 }
 
+/* ---- moving wall ---------------------------------------------------------------------------- */
+
+fe_status fe_wall_init(fe_wall* w, const float min[3], const float max[3]) {
+    if (!w || !min || !max) return fail(FE_ERR_INVALID_ARGUMENT, "null argument");
+    // CollisionBox::new (src/entities/collision.rs:20-29)
+    for (int a = 0; a < 3; a++) {
+        w->min[a] = min[a];
+        w->max[a] = max[a];
+    }
+    w->wave_amplitude = 0.0f;
+    w->wave_frequency = 1.0f;
+    w->time = 0.0f;
+    w->base_min_x = min[0];
+    return FE_OK;
+}
+
+fe_status fe_wall_update(fe_wall* w, float dt) {
+    if (!w) return fail(FE_ERR_INVALID_ARGUMENT, "null wall");
+    wall_update(w, dt);
+    return FE_OK;
+}
+
+fe_status fe_set_wall(fe_ctx* c, const fe_wall* wall) {
+    FE_CHECK_CTX(c);
+    if (c->nscenes > 1) return fail(FE_ERR_UNSUPPORTED, "a batch takes per-scene walls (fe_batch_set_wall)");
+    return set_wall(c, 0, wall);
+}
+
+fe_status fe_get_wall(const fe_ctx* c, fe_wall* out, int* attached) {
+    FE_CHECK_CTX(c);
+    if (c->nscenes > 1) return fail(FE_ERR_UNSUPPORTED, "a batch has per-scene walls (fe_batch_get_wall)");
+    return get_wall(c, 0, out, attached);
+}
+
 /* ---- batched scenes ----------------------------------------------------------------------- */
 
 fe_status fe_batch_create(const fe_params* params, const float* const* xyz, const uint32_t* counts, uint32_t scenes,
@@ -1152,6 +1288,20 @@ fe_status fe_batch_iterations(fe_ctx* c, uint32_t scene, uint32_t* density, uint
     if (density_total) *density_total = c->scene_total_iters[0][scene];
     if (divergence_total) *divergence_total = c->scene_total_iters[1][scene];
     return FE_OK;
+}
+
+fe_status fe_batch_set_wall(fe_ctx* c, uint32_t scene, const fe_wall* wall) {
+    FE_CHECK_CTX(c);
+    fe_status s = check_scene(c, scene);
+    if (s != FE_OK) return s;
+    return set_wall(c, scene, wall);
+}
+
+fe_status fe_batch_get_wall(const fe_ctx* c, uint32_t scene, fe_wall* out, int* attached) {
+    FE_CHECK_CTX(c);
+    fe_status s = check_scene(c, scene);
+    if (s != FE_OK) return s;
+    return get_wall(c, scene, out, attached);
 }
 
 fe_status fe_batch_use_graph(fe_ctx* c, int enable) {

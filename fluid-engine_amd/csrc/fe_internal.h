@@ -17,6 +17,7 @@ constexpr uint32_t kBlock = 256;          // threads per block of the per-partic
 constexpr uint32_t kMaxSolveIters = 1024; // residual slots per solve
 constexpr uint32_t kStatsBlocks = 1024;   // blocks of the stats reduction
 constexpr uint32_t kMaxNbr = 64;          // neighbour-list slots per particle
+constexpr uint32_t kTabRing = 4;          // pinned per-frame scene tables of a batch with moving walls
 // schedule-ordered mirrors of the fields the gathers read from neighbours, and of the
 // per-particle solver fields the Jacobi sweeps read for their own particle
 constexpr uint32_t kMirrorPos = 1u, kMirrorVelB = 2u, kMirrorVelA = 4u, kMirrorDens = 8u, kMirrorPrho = 16u,
@@ -152,6 +153,15 @@ struct fe_ctx {
     bool batch_graph = false;                   // replay the frame from a captured graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+
+    // Moving walls (one per scene) applied at the start of every frame. A batch whose walls
+    // move re-copies its scene constants each frame from a ring of pinned host tables.
+    std::vector<fe_wall> walls;
+    std::vector<uint8_t> wall_on;
+    uint32_t walls_on = 0;
+    fe::Sph* tab_ring = nullptr;                // pinned, kTabRing x nscenes
+    hipEvent_t tab_event[fe::kTabRing] = {};
+    uint32_t tab_slot = 0;
 
     // GpuPhysicsData fields
     float4* pos_a = nullptr;

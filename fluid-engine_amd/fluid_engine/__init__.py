@@ -17,14 +17,15 @@ from . import scene, tuning
 from ._ffi import (EXPORTED_SYMBOLS, FE_MLP_HIDDEN, FE_MLP_INPUTS, FE_MLP_NORM, FE_MLP_PARAMS, FE_NET_DENSITY,
                    FE_NET_DIVERGENCE, FE_SOLVER_CONVERGE, FE_SOLVER_FIXED, FE_SORT_BITONIC, FE_SORT_RADIX,
                    FE_WARM_START_LEARNED, FE_WARM_START_REFERENCE, FIELDS, INT_FIELDS, LIB_PATH, FeOptions,
-                   FeParams, FeTuneEntry, FluidEngineError, check, load_library)
+                   FeParams, FeTuneEntry, FeWall, FluidEngineError, check, load_library)
+from .collision import CollisionBox
 
 __all__ = [
     "Simulation", "FeParams", "FeOptions", "FluidEngineError", "load_library", "scene", "frame_control",
     "default_options", "FE_SORT_BITONIC", "FE_SORT_RADIX", "FE_SOLVER_FIXED", "FE_SOLVER_CONVERGE",
     "FE_WARM_START_REFERENCE", "FE_WARM_START_LEARNED", "FIELDS", "EXPORTED_SYMBOLS", "LIB_PATH",
     "FE_NET_DENSITY", "FE_NET_DIVERGENCE", "FE_MLP_INPUTS", "FE_MLP_HIDDEN", "FE_MLP_PARAMS", "FE_MLP_NORM",
-    "Batch",
+    "Batch", "CollisionBox", "FeWall",
 ]
 
 
@@ -182,6 +183,19 @@ class Simulation:
     def run(self, frames: int, max_dt: float | None = None):
         dt = self.params.dt if max_dt is None else max_dt
         check(self._lib.fe_run(self._ctx, c_float(dt), frames), "fe_run")
+
+    # -- moving wall (CollisionBox) ---------------------------------------------------------
+    def set_wall(self, box: CollisionBox | None):
+        """Attach a copy of `box` (None detaches): every later frame takes the parameter box
+        from the wall, then advances the wall by the frame's max_dt."""
+        check(self._lib.fe_set_wall(self._ctx, None if box is None else byref(box.wall())), "fe_set_wall")
+
+    def wall(self) -> CollisionBox | None:
+        """The attached wall as advanced so far, or None."""
+        w = FeWall()
+        on = ctypes.c_int()
+        check(self._lib.fe_get_wall(self._ctx, byref(w), byref(on)), "fe_get_wall")
+        return CollisionBox.from_wall(w) if on.value else None
 
     # -- readback / timing ----------------------------------------------------------------
     def read_stats(self) -> np.ndarray:
@@ -391,3 +405,13 @@ class Batch(Simulation):
 
     def use_graph(self, enable: bool = True):
         check(self._lib.fe_batch_use_graph(self._ctx, 1 if enable else 0), "fe_batch_use_graph")
+
+    def set_scene_wall(self, scene: int, box: CollisionBox | None):
+        check(self._lib.fe_batch_set_wall(self._ctx, scene, None if box is None else byref(box.wall())),
+              "fe_batch_set_wall")
+
+    def scene_wall(self, scene: int) -> CollisionBox | None:
+        w = FeWall()
+        on = ctypes.c_int()
+        check(self._lib.fe_batch_get_wall(self._ctx, scene, byref(w), byref(on)), "fe_batch_get_wall")
+        return CollisionBox.from_wall(w) if on.value else None

@@ -93,7 +93,21 @@ class FeOptions(Structure):
     ]
 
 
+class FeWall(Structure):
+    """`CollisionBox` (src/entities/collision.rs:11-18) with its private time and base_min_x."""
+
+    _fields_ = [
+        ("min", c_float * 3),
+        ("max", c_float * 3),
+        ("wave_amplitude", c_float),
+        ("wave_frequency", c_float),
+        ("time", c_float),
+        ("base_min_x", c_float),
+    ]
+
+
 assert ctypes.sizeof(FeParams) == 112
+assert ctypes.sizeof(FeWall) == 40
 
 
 class FluidEngineError(RuntimeError):
@@ -147,6 +161,10 @@ _SIGNATURES = [
     ("fe_profile_reset", [c_void_p]),
     ("fe_frame_control", [POINTER(c_uint32), c_uint32, POINTER(FeParams), POINTER(FeOptions),
                           POINTER(c_float), POINTER(c_uint32), POINTER(c_uint32)]),
+    ("fe_wall_init", [POINTER(FeWall), POINTER(c_float), POINTER(c_float)]),
+    ("fe_wall_update", [POINTER(FeWall), c_float]),
+    ("fe_set_wall", [c_void_p, POINTER(FeWall)]),
+    ("fe_get_wall", [c_void_p, POINTER(FeWall), POINTER(c_int)]),
     ("fe_learn_collect", [c_void_p, c_uint32, c_uint32]),
     ("fe_learn_dataset_size", [c_void_p, c_int, POINTER(c_uint32)]),
     ("fe_learn_dataset", [c_void_p, c_int, c_void_p, c_void_p, c_uint32]),
@@ -167,6 +185,8 @@ _SIGNATURES = [
     ("fe_batch_iterations", [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint64),
                              POINTER(c_uint64)]),
     ("fe_batch_use_graph", [c_void_p, c_int]),
+    ("fe_batch_set_wall", [c_void_p, c_uint32, POINTER(FeWall)]),
+    ("fe_batch_get_wall", [c_void_p, c_uint32, POINTER(FeWall), POINTER(c_int)]),
     ("fe_autotune", [c_void_p, c_uint32]),
     ("fe_tune_count", [c_void_p, POINTER(c_uint32)]),
     ("fe_tune_get", [c_void_p, c_uint32, POINTER(FeTuneEntry)]),

@@ -85,6 +85,8 @@ class Workload:
     name: str
     positions: np.ndarray
     params: FeParams
+    # (amplitude m, frequency Hz) of the box's moving min.x wall, or None for a fixed box
+    wall_wave: tuple | None = None
 
     @property
     def n(self) -> int:
@@ -133,6 +135,20 @@ def rest_density_scene() -> Workload:
     params = make_params(float(r), float(mass), float(h), 1000.0, 0.15, 0.5, 0.005, 4, 4, (0.0, -9.81, 0.0),
                          (-1.5, 0.0, -1.0), (0.8, 4.0, 1.0))
     return Workload("rest_density_4800", pos, params)
+
+
+def app_scene(seed=0) -> Workload:
+    """Scene::new (src/core/scene.rs:14-86), the scene the reference app opens: a 1.0 x 2.0 x
+    0.8 m block spawned at (-1, 1, -0.8), r = 0.02, s = 0.04 (25 x 50 x 20 = 25000 particles),
+    jitter 0.01, h = 0.08, dt = 5 ms, viscosity 0.15, relax 0.5, 4/4 sweeps, box
+    (-1.5, 0, -1) .. (0.8, 4, 1) whose min.x wall oscillates with amplitude 0.3 m at 0.5 Hz."""
+    r = np.float32(0.02)
+    s = r * np.float32(2.0)
+    h = r * np.float32(4.0)
+    pos, mass = generate_volume((-1.0, 1.0, -0.8), 1.0, 2.0, 0.8, r, 1000.0, s, 0.01, seed=seed)
+    params = make_params(float(r), float(mass), float(h), 1000.0, 0.15, 0.5, 0.005, 4, 4, (0.0, -9.81, 0.0),
+                         (-1.5, 0.0, -1.0), (0.8, 4.0, 1.0))
+    return Workload("app_scene_25000", pos, params, wall_wave=(0.3, 0.5))
 
 
 def batch_scenes(count=64, seed=0, n_min=4096, n_max=262144, **kw) -> list:

@@ -211,6 +211,37 @@ fe_status fe_frame_control(const uint32_t stats[3], uint32_t n, const fe_params*
                            const fe_options* options, float* dt_out,
                            uint32_t* density_iters, uint32_t* divergence_iters);
 
+/* ---- moving wall: `CollisionBox` (src/entities/collision.rs:11-35) ---------------------------
+ * The simulation box; its min.x wall oscillates about base_min_x. Each reference frame first
+ * copies the box into the parameter block in the UI pass (src/renderer/ui.rs:125-126:
+ * box_min = (min, 0), box_max = (max, 0)), runs the physics with it, and advances the wall
+ * by max_dt after uploading the parameters (src/renderer/mod.rs:269-270). Frame k therefore
+ * runs with the wall at time t_k, the float sum of the k earlier max_dt (t_0 = 0 gives
+ * min.x = base_min_x as set by fe_wall_init). The scene of the app sets amplitude 0.3 and
+ * frequency 0.5 (src/core/scene.rs:19-23). */
+typedef struct fe_wall {
+    float min[3];
+    float max[3];
+    float wave_amplitude;   /* 0 after fe_wall_init (collision.rs:24) */
+    float wave_frequency;   /* 1 after fe_wall_init (collision.rs:25) */
+    float time;             /* seconds advanced so far (private in the reference) */
+    float base_min_x;       /* min[0] at fe_wall_init (private in the reference) */
+} fe_wall;
+/* CollisionBox::new (collision.rs:20-29). Pure host function. */
+fe_status fe_wall_init(fe_wall* wall, const float min[3], const float max[3]);
+/* CollisionBox::update (collision.rs:31-35): time += dt; min[0] = base_min_x +
+ * wave_amplitude * sin(time * wave_frequency * 2 pi), each operation in float, sin being the
+ * C library's sinf (which Rust's f32::sin calls). Pure host function. */
+fe_status fe_wall_update(fe_wall* wall, float dt);
+/* Attach a copy of `wall` to a single-scene context (NULL detaches; every field must be
+ * finite). While one is attached, each fe_step / fe_run frame writes the wall's box into the
+ * parameters before the frame (replacing the box fe_set_params gave) and advances the wall by
+ * the frame's max_dt after it. */
+fe_status fe_set_wall(fe_ctx* ctx, const fe_wall* wall);
+/* The attached wall as advanced by the frames run since; *attached (may be NULL) is 0 and
+ * *out zeroed when none is attached. */
+fe_status fe_get_wall(const fe_ctx* ctx, fe_wall* out, int* attached);
+
 /* ---- launch-configuration autotuner (no reference counterpart) --------------------------------
  * Tunable kernels: the six list gathers (density_alpha, viscosity, density_source,
  * divergence_source, pressure_force, pressure_update: block 64..1024), build_list (block
@@ -278,6 +309,11 @@ fe_status fe_batch_iterations(fe_ctx* ctx, uint32_t scene, uint32_t* density, ui
 /* With FE_SOLVER_FIXED, capture the steady-state frame of the batch once in a HIP graph and
  * replay it for every later frame (parameter, option and field changes drop the graph). */
 fe_status fe_batch_use_graph(fe_ctx* ctx, int enable);
+/* A moving wall per scene, as fe_set_wall / fe_get_wall: the scene's box comes from its
+ * wall at the start of every frame (the per-scene constants are copied to the device on the
+ * context's stream, also before graph replays). */
+fe_status fe_batch_set_wall(fe_ctx* ctx, uint32_t scene, const fe_wall* wall);
+fe_status fe_batch_get_wall(const fe_ctx* ctx, uint32_t scene, fe_wall* out, int* attached);
 
 /* ---- learned warm start (no reference counterpart) -----------------------------------------
  * Two networks, FE_NET_DENSITY and FE_NET_DIVERGENCE, one per Jacobi solve. Each maps the

@@ -115,6 +115,8 @@ struct State {
     uint32_t stats[3] = {0, 0, 0};
     uint32_t last_iters[2] = {0, 0};
     bool have_stats = false;
+    fe_wall wall{};
+    bool wall_on = false;
 
     void alloc(uint32_t n_) {
         n = n_;
@@ -638,11 +640,34 @@ void frame_control(State<R>& s, uint32_t* dit, uint32_t* vit) {
     if (s.o.use_solver_error_threshold && avg_ve < vthr) *vit = 1;
 }
 
-// Renderer::step physics frame (src/renderer/mod.rs:279-342).
+// CollisionBox::update (src/entities/collision.rs:31-35): `self.time += dt; let offset =
+// self.wave_amplitude * (self.time * self.wave_frequency * TAU).sin(); self.min.x =
+// self.base_min_x + offset;` in f32, left to right, TAU = std::f32::consts::TAU.
+void wall_update(fe_wall* w, float dt) {
+    const float tau = (float)(2.0 * 3.14159265358979323846);
+    w->time += dt;
+    float arg = w->time * w->wave_frequency;
+    arg = arg * tau;
+    float offset = w->wave_amplitude * sinf(arg);
+    w->min[0] = w->base_min_x + offset;
+}
+
+// Renderer::step physics frame (src/renderer/mod.rs:279-342). With a moving wall, the UI
+// pass that precedes the frame copies the box into the parameters (src/renderer/ui.rs:
+// 125-126) and the wall advances by max_dt once the parameters are uploaded (mod.rs:269-270).
 template <class R>
 void step(State<R>& s, float max_dt) {
+    if (s.wall_on) {
+        for (int a = 0; a < 3; a++) {
+            s.p.box_min[a] = s.wall.min[a];
+            s.p.box_max[a] = s.wall.max[a];
+        }
+        s.p.box_min[3] = 0.0f;
+        s.p.box_max[3] = 0.0f;
+    }
     uint32_t dit, vit;
     frame_control(s, &dit, &vit);
+    if (s.wall_on) wall_update(&s.wall, max_dt);
     neighbor_search(s);
     density_alpha(s);
     float t = 0.0f;
@@ -875,6 +900,16 @@ uint32_t oracle_solve_divergence(oracle_ctx* c) {
 }
 void oracle_step(oracle_ctx* c, float max_dt) {
     if (c->fp64) step(c->d, max_dt); else step(c->f, max_dt);
+}
+void oracle_wall_update(fe_wall* w, float dt) { wall_update(w, dt); }
+void oracle_set_wall(oracle_ctx* c, const fe_wall* w) {
+    c->f.wall_on = c->d.wall_on = w != nullptr;
+    c->f.wall = c->d.wall = w ? *w : fe_wall{};
+}
+int oracle_get_wall(oracle_ctx* c, fe_wall* out) {
+    const auto& s = c->fp64 ? c->d.wall : c->f.wall;
+    *out = s;
+    return (c->fp64 ? c->d.wall_on : c->f.wall_on) ? 1 : 0;
 }
 void oracle_last_iterations(oracle_ctx* c, uint32_t* density, uint32_t* divergence) {
     const uint32_t* it = c->fp64 ? c->d.last_iters : c->f.last_iters;

@@ -70,6 +70,13 @@ uint32_t oracle_solve_divergence(oracle_ctx* c);
 /* Renderer::step physics frame with the current params and options (host flags from the
  * previous frame's stats when enabled). */
 void oracle_step(oracle_ctx* c, float max_dt);
+/* CollisionBox::update (src/entities/collision.rs:31-35) on a wall; attach a wall (NULL
+ * detaches) that oracle_step copies into the box before each frame and then advances by
+ * max_dt (src/renderer/ui.rs:125-126, src/renderer/mod.rs:269-270); read it back (returns
+ * whether one is attached). */
+void oracle_wall_update(fe_wall* w, float dt);
+void oracle_set_wall(oracle_ctx* c, const fe_wall* w);
+int oracle_get_wall(oracle_ctx* c, fe_wall* out);
 void oracle_last_iterations(oracle_ctx* c, uint32_t* density, uint32_t* divergence);
 
 /* Stand-alone pieces for unit tests. */

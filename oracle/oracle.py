@@ -58,6 +58,9 @@ def lib():
         "oracle_bitonic_sort_pairs": ([c_void_p, c_uint32], None),
         "oracle_residual_q": ([c_float, c_float, c_float], c_uint64),
         "oracle_features": ([P, c_void_p, c_void_p], None),
+        "oracle_wall_update": ([c_void_p, c_float], None),
+        "oracle_set_wall": ([P, c_void_p], None),
+        "oracle_get_wall": ([P, c_void_p], c_int),
     }
     for stage in ("neighbor_search", "sort", "density_alpha", "viscosity", "density_source", "pressure_force",
                   "pressure_integration", "divergence_source", "divergence_integration", "stats"):
@@ -77,6 +80,32 @@ FIELDS = {
     "pressure_accelerations": (9, 4, False), "grid_entries": (10, 2, True), "grid_start": (11, 1, True),
 }
 INT_FIELDS = ("grid_entries", "grid_start")
+
+
+class Wall(ctypes.Structure):
+    """The fe_wall layout of include/fluid_engine.h (CollisionBox, src/entities/collision.rs:11-18)."""
+
+    _fields_ = [("min", c_float * 3), ("max", c_float * 3), ("wave_amplitude", c_float),
+                ("wave_frequency", c_float), ("time", c_float), ("base_min_x", c_float)]
+
+
+def make_wall(box_min, box_max, amplitude=0.0, frequency=1.0):
+    """CollisionBox::new (collision.rs:20-29) followed by setting the two wave fields."""
+    w = Wall()
+    for a in range(3):
+        w.min[a] = box_min[a]
+        w.max[a] = box_max[a]
+    w.wave_amplitude = amplitude
+    w.wave_frequency = frequency
+    w.time = 0.0
+    w.base_min_x = box_min[0]
+    return w
+
+
+def wall_update(wall, dt):
+    """CollisionBox::update (collision.rs:31-35), in place."""
+    lib().oracle_wall_update(ctypes.addressof(wall), c_float(dt))
+    return wall
 
 
 def set_threads(t: int):
@@ -163,6 +192,16 @@ class Oracle:
 
     def step(self, max_dt=None):
         self._l.oracle_step(self._c, c_float(self._params.dt if max_dt is None else max_dt))
+
+    def set_wall(self, wall):
+        """Attach a moving wall (None detaches): each step copies its box into the
+        parameters, then advances it by max_dt."""
+        self._l.oracle_set_wall(self._c, None if wall is None else ctypes.addressof(wall))
+
+    def wall(self):
+        w = Wall()
+        on = self._l.oracle_get_wall(self._c, ctypes.addressof(w))
+        return w if on else None
 
     def read_stats(self):
         out = (c_uint32 * 3)()
