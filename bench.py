@@ -50,6 +50,14 @@ ALGO_BYTES["divergence_source_feat"] = ALGO_BYTES["divergence_source"] + 52.0
 # cores (BF16 dense peak 2.5 PFLOP/s).
 MLP_FLOP = 2.0 * (12 * 64 + 2 * 64 * 64 + 64)
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+# The splat gather's stencil: 1647 offsets with |o|^2 <= 53 (the non-zero contributions),
+# formed for every texel of a 32 x 8 x 4 tile whose halo holds a particle. Per lane (4 texels)
+# and busy tile it reads 2178 LDS words and stores its share of the 46 x 22 x 18-word halo.
+SPLAT_TAPS = 1647
+SPLAT_TILE = (32, 8, 4)
+SPLAT_LDS_WORDS_PER_LANE = 2178 + 46 * 22 * 18 / 256
+# LDS read peak: 128 B/clk per CU (the CDNA figure) x 256 CUs x 2.4 GHz.
+LDS_PEAK_TBS = 128 * 256 * 2.4e9 / 1e12
 
 
 def parse():
@@ -68,6 +76,7 @@ def parse():
     ap.add_argument("--tune-cache", default=None, help="JSON cache of tuned configurations (read and written)")
     ap.add_argument("--batch-frames", type=int, default=10)
     ap.add_argument("--no-sort-bench", action="store_true", help="skip the isolated 2^22 sort measurement")
+    ap.add_argument("--no-splat", action="store_true", help="skip the density-splat (render hand-off) measurement")
     ap.add_argument("--learned-collect", type=int, default=100, help="frames sampled before training")
     ap.add_argument("--learned-eval", type=int, default=400, help="frames compared after training")
     ap.add_argument("--copy-probe", action="store_true",
@@ -241,6 +250,8 @@ def main():
             "configs": {k: [v["block"], v["items"]] for k, v in tuned.items()},
             "median_us_tuned_vs_default": {k: [v["median_us"], v["default_us"]] for k, v in tuned.items()}},
     }
+    if not args.no_splat:
+        out["render_handoff"] = splat_measurement(fe, sim, w, args.no_cpu_baseline)
     sim.close()
     if not args.no_learned and args.solver == "converge":
         out["learned_warm_start"] = learned_comparison(fe, w, opts, args.learned_collect, args.learned_eval, tuned)
@@ -283,6 +294,93 @@ def sort_measurement(fe, scene, log2n=22, warmup=3, reps=15):
                         "measures": "submit-to-fence wall clock", "source": "scripts/sort_benchmark.csv:20-21"}
     # 68 algorithmic bytes per pair for a four-pass LSD sort of 8-byte pairs, at 8 TB/s
     out["radix_hbm_floor_ms"] = round(68 * n / 8e12 * 1e3, 4)
+    return out
+
+
+def splat_work(pos, params):
+    """What one splat of these positions costs the reference and this build: the reference's
+    image atomics (texels at d^2 < 64 inside the image around each in-box particle's centre
+    texel, splat_density.comp:29-47) and the gather's busy 32 x 8 x 4 tiles (a tile works
+    when an 8^3 block of centre texels within 7 texels of it holds a particle)."""
+    import numpy as np
+    from scipy.signal import fftconvolve
+    f = np.float32
+    res = np.array([int(params.grid_res[a]) for a in range(3)])
+    mn = np.array(params.box_min[:3], f)
+    inv = (f(1.0) / (np.array(params.box_max[:3], f) - mn)).astype(f)
+    uvw = ((pos[:, :3].astype(f) - mn) * inv).astype(f)
+    ok = np.all((uvw >= 0) & (uvw <= 1), axis=1)
+    c = (uvw[ok] * res.astype(f)).astype(np.int64)
+    cnt = np.zeros(tuple(res + 1), np.float64)
+    np.add.at(cnt, (c[:, 0], c[:, 1], c[:, 2]), 1.0)
+    o = np.arange(-8, 9)
+    ball = ((o[:, None, None] ** 2 + o[None, :, None] ** 2 + o[None, None, :] ** 2) < 64).astype(np.float64)
+    atomics = float(np.rint(fftconvolve(cnt, ball, mode="full")[8:8 + res[0], 8:8 + res[1], 8:8 + res[2]]).sum())
+    occ = np.zeros(tuple(res // 8 + 1), bool)
+    occ[c[:, 0] // 8, c[:, 1] // 8, c[:, 2] // 8] = True
+    tiles = [(res[a] + SPLAT_TILE[a] - 1) // SPLAT_TILE[a] for a in range(3)]
+    busy = 0
+    for tz in range(tiles[2]):
+        z0 = tz * SPLAT_TILE[2]
+        bz = slice(max(z0 - 7, 0) // 8, min(z0 + SPLAT_TILE[2] - 1 + 7, res[2]) // 8 + 1)
+        for ty in range(tiles[1]):
+            y0 = ty * SPLAT_TILE[1]
+            by = slice(max(y0 - 7, 0) // 8, min(y0 + SPLAT_TILE[1] - 1 + 7, res[1]) // 8 + 1)
+            for tx in range(tiles[0]):
+                x0 = tx * SPLAT_TILE[0]
+                bx = slice(max(x0 - 7, 0) // 8, min(x0 + SPLAT_TILE[0] - 1 + 7, res[0]) // 8 + 1)
+                busy += bool(occ[bx, by, bz].any())
+    return {"in_box": int(ok.sum()), "reference_atomics": atomics, "busy_tiles": busy,
+            "tiles": int(np.prod(tiles))}
+
+
+def splat_measurement(fe, sim, w, no_cpu, warmup=3, reps=20):
+    """The render hand-off on the benchmark's state (after the timed frames): clear + splat of
+    the raymarching density texture (grid_res^3 uint32) from position_a, timed with HIP events
+    (the mean over `reps`), with per-kernel durations, the gather's integer multiply-add issue
+    rate against the VALU peak, and the oracle's splat on the host cores."""
+    for _ in range(warmup):
+        sim.splat_density()
+    sim.sync()
+    sim.timer_start()
+    for _ in range(reps):
+        sim.splat_density()
+    ms = sim.timer_stop() / reps
+    sim.profile(True)
+    sim.profile_reset()
+    for _ in range(reps):
+        sim.splat_density()
+    prof = sim.profile_results()
+    sim.profile(False)
+    kernels = {k: round(v[1] * 1e3 / max(v[0], 1), 3) for k, v in prof.items() if k.startswith("splat")}
+    pos = sim.download("position_a")
+    work = splat_work(pos, sim.params)
+    gather_s = kernels["splat_gather"] * 1e-6
+    lanes = work["busy_tiles"] * SPLAT_TILE[0] * SPLAT_TILE[1]
+    lds_bytes = lanes * SPLAT_LDS_WORDS_PER_LANE * 4
+    texel_taps = lanes * SPLAT_TILE[2] * SPLAT_TAPS
+    res = [int(sim.params.grid_res[a]) for a in range(3)]
+    out = {"texture": f"{res[0]}x{res[1]}x{res[2]} uint32", "particles": w.n, "ms_per_splat": round(ms, 4),
+           "kernels_avg_us": kernels, "reference_atomics_per_splat": work["reference_atomics"],
+           "reference_atomics_per_s": round(work["reference_atomics"] / (ms * 1e-3), 1),
+           "busy_tiles": work["busy_tiles"], "tiles": work["tiles"],
+           "roofline": {"bound": "lds", "kernel": "splat_gather", "achieved": round(lds_bytes / gather_s / 1e12, 2),
+                        "peak": round(LDS_PEAK_TBS, 2), "unit": "TB/s",
+                        "frac": round(lds_bytes / gather_s / 1e12 / LDS_PEAK_TBS, 4),
+                        "lds_bytes_per_launch": round(lds_bytes), "texel_taps_per_s": round(texel_taps / gather_s, 1),
+                        "avg_launch_us": kernels["splat_gather"]}}
+    if not no_cpu:
+        from oracle import oracle
+        cores = min(os.cpu_count() or 1, 16)
+        oracle.set_threads(cores)
+        orc = oracle.Oracle(sim.params, w.positions)
+        orc.upload("position_a", pos)
+        t0 = time.perf_counter()
+        orc.splat_density()
+        el = time.perf_counter() - t0
+        orc.close()
+        out["cpu_baseline"] = {"value": round(1e3 * el, 1), "unit": "ms per splat", "cores": cores, "kind": "port",
+                               "sample": "one splat of the same position_a by the oracle (OpenMP, private textures)"}
     return out
 
 

@@ -68,6 +68,7 @@ void free_ctx(fe_ctx* c) {
         if (p) (void)hipFree(p);
     fe::sort_scratch_free(&c->sort);
     fe::learn_free(c);
+    fe::splat_free(c);
     for (void* p : {(void*)c->scene_tab, (void*)c->sid, (void*)c->scene_stats})
         if (p) (void)hipFree(p);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -1039,6 +1040,59 @@ fe_status fe_frame_control(const uint32_t stats[3], uint32_t n, const fe_params*
     if (dt_out) *dt_out = p.dt;
     if (density_iters) *density_iters = dit;
     if (divergence_iters) *divergence_iters = vit;
+    return FE_OK;
+}
+
+/* ---- render hand-off ------------------------------------------------------------------------- */
+
+fe_status fe_splat_density(fe_ctx* c) {
+    FE_CHECK_CTX(c);
+    if (c->nscenes > 1) return fail(FE_ERR_UNSUPPORTED, "a batch has no density texture");
+    uint64_t centres = 1;
+    for (int a = 0; a < 3; a++) {
+        if (c->params.grid_res[a] < 1 || c->params.grid_res[a] > 2048)
+            return fail(FE_ERR_INVALID_ARGUMENT, "grid_res must be in [1, 2048] per axis");
+        centres *= (uint64_t)c->params.grid_res[a] + 1;
+    }
+    if (centres > (1ull << 31)) return fail(FE_ERR_INVALID_ARGUMENT, "grid_res too large: (x+1)(y+1)(z+1) > 2^31");
+    DeviceGuard g(c->device);
+    c->splat.valid = false;
+    FE_HIP(fe::launch_splat_density(c), "splat_density");
+    c->splat.valid = true;
+    return FE_OK;
+}
+
+fe_status fe_density_texture(fe_ctx* c, void** device_ptr, size_t* bytes) {
+    FE_CHECK_CTX(c);
+    if (!c->splat.valid) return fail(FE_ERR_UNSUPPORTED, "no density texture: call fe_splat_density first");
+    if (device_ptr) *device_ptr = c->splat.tex;
+    if (bytes) *bytes = 4 * c->splat.cells;
+    return FE_OK;
+}
+
+fe_status fe_download_density_texture(fe_ctx* c, uint32_t* dst, size_t bytes) {
+    FE_CHECK_CTX(c);
+    if (!c->splat.valid) return fail(FE_ERR_UNSUPPORTED, "no density texture: call fe_splat_density first");
+    if (bytes != 4 * c->splat.cells)
+        return fail(FE_ERR_SIZE_MISMATCH, "density texture is " + std::to_string(4 * c->splat.cells) + " bytes");
+    if (!dst) return fail(FE_ERR_INVALID_ARGUMENT, "null destination");
+    DeviceGuard g(c->device);
+    FE_HIP(hipMemcpyAsync(dst, c->splat.tex, bytes, hipMemcpyDeviceToHost, c->stream), "texture download");
+    FE_HIP(hipStreamSynchronize(c->stream), "texture download wait");
+    return FE_OK;
+}
+
+fe_status fe_copy_field_device(fe_ctx* c, int field, void* dst, size_t bytes) {
+    FE_CHECK_CTX(c);
+    size_t want;
+    void* dev;
+    if (!field_info(c, field, &want, &dev)) return fail(FE_ERR_INVALID_ARGUMENT, "unknown field");
+    if (!dev) return fail(FE_ERR_UNSUPPORTED, "grid fields have no device layout of the reference's");
+    if (bytes != want)
+        return fail(FE_ERR_SIZE_MISMATCH, "field size mismatch: expected " + std::to_string(want) + " bytes");
+    if (!dst) return fail(FE_ERR_INVALID_ARGUMENT, "null destination");
+    DeviceGuard g(c->device);
+    FE_HIP(hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToDevice, c->stream), "device copy");
     return FE_OK;
 }
 

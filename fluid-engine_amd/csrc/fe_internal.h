@@ -102,6 +102,15 @@ struct LearnState {
     size_t io_floats = 0;
 };
 
+// The density texture of the render hand-off and its scratch (allocated on first use).
+struct SplatState {
+    uint32_t* tex = nullptr;     // grid_res.x * .y * .z texels, x fastest
+    uint32_t* count = nullptr;   // (grid_res + 1)^3 centre counts, then the 8^3 occupancy blocks
+    size_t cells = 0, centres = 0;
+    bool checked = false;        // the kernel's constant weights equal the host evaluation
+    bool valid = false;          // a splat has run since the last reallocation
+};
+
 struct Launch;
 struct Sph;
 
@@ -162,6 +171,8 @@ struct fe_ctx {
     fe::Sph* tab_ring = nullptr;                // pinned, kTabRing x nscenes
     hipEvent_t tab_event[fe::kTabRing] = {};
     uint32_t tab_slot = 0;
+
+    fe::SplatState splat;
 
     // GpuPhysicsData fields
     float4* pos_a = nullptr;
@@ -351,5 +362,12 @@ void learn_features_done(fe_ctx* c, int kind);
 // the summed squared error into loss_out (if not null); applies Adam when apply != 0.
 hipError_t learn_step(fe_ctx* c, int which, const float* x, const float* t, uint32_t batch, uint32_t sample_mod,
                       uint32_t seed, float lr, uint32_t apply, float* grad_out, float* loss_out);
+
+// ---- launchers (fe_splat.hip) ----
+// Clear and rebuild the density texture from position_a with the current box and grid_res.
+hipError_t launch_splat_density(fe_ctx* c);
+// Contribution of a texel at squared offset d2 (0..63) from a particle's centre texel.
+void splat_weights(uint32_t wt[64]);
+void splat_free(fe_ctx* c);
 
 }  // namespace fe

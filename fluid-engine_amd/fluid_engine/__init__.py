@@ -184,6 +184,35 @@ class Simulation:
         dt = self.params.dt if max_dt is None else max_dt
         check(self._lib.fe_run(self._ctx, c_float(dt), frames), "fe_run")
 
+    # -- render hand-off ---------------------------------------------------------------------
+    def splat_density(self):
+        """Clear and rebuild the density texture (splat_density.comp) from position_a."""
+        check(self._lib.fe_splat_density(self._ctx), "fe_splat_density")
+
+    def density_texture(self) -> np.ndarray:
+        """The last splat's texture as uint32 [grid_res.z, grid_res.y, grid_res.x]."""
+        ptr = c_void_p()
+        size = ctypes.c_size_t()
+        check(self._lib.fe_density_texture(self._ctx, byref(ptr), byref(size)), "fe_density_texture")
+        p = self.params
+        shape = (int(p.grid_res[2]), int(p.grid_res[1]), int(p.grid_res[0]))
+        out = np.empty(size.value // 4, dtype=np.uint32)
+        check(self._lib.fe_download_density_texture(self._ctx, out.ctypes.data, out.nbytes),
+              "fe_download_density_texture")
+        return out.reshape(shape) if out.size == shape[0] * shape[1] * shape[2] else out
+
+    def density_texture_ptr(self):
+        """(device address, bytes) of the last splat's texture."""
+        ptr = c_void_p()
+        size = ctypes.c_size_t()
+        check(self._lib.fe_density_texture(self._ctx, byref(ptr), byref(size)), "fe_density_texture")
+        return ptr.value, size.value
+
+    def copy_field_device(self, name: str, dst_ptr: int, nbytes: int):
+        """Enqueue a device-to-device copy of a particle field into caller memory."""
+        fid = FIELDS[name][0]
+        check(self._lib.fe_copy_field_device(self._ctx, fid, c_void_p(dst_ptr), nbytes), "fe_copy_field_device")
+
     # -- moving wall (CollisionBox) ---------------------------------------------------------
     def set_wall(self, box: CollisionBox | None):
         """Attach a copy of `box` (None detaches): every later frame takes the parameter box

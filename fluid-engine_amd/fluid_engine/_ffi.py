@@ -161,6 +161,10 @@ _SIGNATURES = [
     ("fe_profile_reset", [c_void_p]),
     ("fe_frame_control", [POINTER(c_uint32), c_uint32, POINTER(FeParams), POINTER(FeOptions),
                           POINTER(c_float), POINTER(c_uint32), POINTER(c_uint32)]),
+    ("fe_splat_density", [c_void_p]),
+    ("fe_density_texture", [c_void_p, POINTER(c_void_p), POINTER(c_size_t)]),
+    ("fe_download_density_texture", [c_void_p, c_void_p, c_size_t]),
+    ("fe_copy_field_device", [c_void_p, c_int, c_void_p, c_size_t]),
     ("fe_wall_init", [POINTER(FeWall), POINTER(c_float), POINTER(c_float)]),
     ("fe_wall_update", [POINTER(FeWall), c_float]),
     ("fe_set_wall", [c_void_p, POINTER(FeWall)]),

@@ -211,6 +211,27 @@ fe_status fe_frame_control(const uint32_t stats[3], uint32_t n, const fe_params*
                            const fe_options* options, float* dt_out,
                            uint32_t* density_iters, uint32_t* divergence_iters);
 
+/* ---- render hand-off (src/renderer/mod.rs:344-355, :385-391) ----------------------------------
+ * The density texture of the raymarching view, `GpuSceneResources::density_texture`
+ * (src/renderer/resources.rs:55-69): R32Uint, grid_res.x x grid_res.y x grid_res.z, held by
+ * the context as uint32 with x fastest, texel (x, y, z) at (z * grid_res.y + y) * grid_res.x
+ * + x. fe_splat_density enqueues the clear to 0 (mod.rs:386-388) and splat_density.comp
+ * (shaders/compute/splat_density.comp:11-51) over position_a with the current parameter
+ * block's box and grid_res (1..2048 per axis, (x+1)(y+1)(z+1) <= 2^31): every particle with
+ * uvw = (pos - box_min) / (box_max - box_min) in [0, 1]^3 adds uint(200 (1 - (d / 8)^2)^3) to
+ * each texel at distance d < 8 texels from its centre texel ivec3(uvw * grid_res).
+ * Single-scene contexts only. */
+fe_status fe_splat_density(fe_ctx* ctx);
+/* Device address and size of the texture of the last fe_splat_density (zero-copy interop;
+ * valid until the next splat with another grid_res). */
+fe_status fe_density_texture(fe_ctx* ctx, void** device_ptr, size_t* bytes);
+/* Copy the texture to host memory (bytes must equal its size). Waits for the stream. */
+fe_status fe_download_density_texture(fe_ctx* ctx, uint32_t* dst, size_t bytes);
+/* copy_buffer of a particle field into caller device memory, enqueued on the context's
+ * stream (mod.rs:347-355 copies position_a and colors into the render buffers of the next
+ * frame in flight). bytes must equal the field's size; grid fields are refused. */
+fe_status fe_copy_field_device(fe_ctx* ctx, int field, void* dst_device, size_t bytes);
+
 /* ---- moving wall: `CollisionBox` (src/entities/collision.rs:11-35) ---------------------------
  * The simulation box; its min.x wall oscillates about base_min_x. Each reference frame first
  * copies the box into the parameter block in the UI pass (src/renderer/ui.rs:125-126:

@@ -688,6 +688,66 @@ void step(State<R>& s, float max_dt) {
 
 }  // namespace
 
+// splat_density.comp:11-51 after the texture clear of src/renderer/mod.rs:386-388, over
+// position_a with the parameter block's box and grid_res (density_texture.rs:30-55 binds
+// position_a, the R32Uint image and sim_params). The texture is grid_res.x * grid_res.y *
+// grid_res.z uint32, x fastest. Every particle whose uvw lies in [0, 1]^3 adds
+// uint(weight * 200) for each texel of the 17^3 cube around its centre texel that is inside
+// the image and at distance < 8. uvw = (pos - box_min) / box_size is evaluated as
+// (pos - box_min) * (1 / box_size), the form the reference GPU used for the hash's division by
+// a uniform (see cell_of). A particle with a NaN uvw (undefined in the shader: every
+// comparison is false and ivec3(NaN) has no value) is skipped. Each thread accumulates a
+// private texture that is summed at the end: unsigned addition modulo 2^32 is the atomic's.
+template <class R>
+void splat_density(const State<R>& s, uint32_t* out) {
+    const int32_t rx = s.p.grid_res[0], ry = s.p.grid_res[1], rz = s.p.grid_res[2];
+    const size_t cells = (size_t)rx * (size_t)ry * (size_t)rz;
+    std::fill(out, out + cells, 0u);
+    float inv[3], mn[3];
+    for (int a = 0; a < 3; a++) {
+        mn[a] = s.p.box_min[a];
+        inv[a] = 1.0f / (s.p.box_max[a] - s.p.box_min[a]);
+    }
+    const int radius = 8;
+    const float r = (float)radius;
+    const float res[3] = {(float)rx, (float)ry, (float)rz};
+#pragma omp parallel
+    {
+        std::vector<uint32_t> mine(cells, 0u);
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < (int64_t)s.n; i++) {
+            float uvw[3];
+            bool nan = false, outside = false;
+            for (int a = 0; a < 3; a++) {
+                const float pa = (float)s.pos_a[4 * (size_t)i + a];
+                uvw[a] = (pa - mn[a]) * inv[a];
+                nan |= std::isnan(uvw[a]);
+                outside |= uvw[a] < 0.0f || uvw[a] > 1.0f;
+            }
+            if (outside || nan) continue;
+            const int32_t cx = (int32_t)(uvw[0] * res[0]), cy = (int32_t)(uvw[1] * res[1]),
+                          cz = (int32_t)(uvw[2] * res[2]);
+            for (int z = -radius; z <= radius; z++)
+                for (int y = -radius; y <= radius; y++)
+                    for (int x = -radius; x <= radius; x++) {
+                        const int32_t vx = cx + x, vy = cy + y, vz = cz + z;
+                        if (vx < 0 || vy < 0 || vz < 0 || vx >= rx || vy >= ry || vz >= rz) continue;
+                        const float fx = (float)x, fy = (float)y, fz = (float)z;
+                        const float dist = std::sqrt(fx * fx + fy * fy + fz * fz);
+                        if (dist < r) {
+                            const float xn = dist / r;
+                            float w = 1.0f - xn * xn;
+                            w = w * w * w;
+                            const uint32_t add = (uint32_t)(w * 200.0f);
+                            mine[((size_t)vz * ry + vy) * rx + vx] += add;
+                        }
+                    }
+        }
+#pragma omp critical
+        for (size_t k = 0; k < cells; k++) out[k] += mine[k];
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // C interface.
 
@@ -900,6 +960,14 @@ uint32_t oracle_solve_divergence(oracle_ctx* c) {
 }
 void oracle_step(oracle_ctx* c, float max_dt) {
     if (c->fp64) step(c->d, max_dt); else step(c->f, max_dt);
+}
+int oracle_splat_density(oracle_ctx* c, uint32_t* out, size_t cells) {
+    const fe_params& p = c->fp64 ? c->d.p : c->f.p;
+    for (int a = 0; a < 3; a++)
+        if (p.grid_res[a] < 1) return 1;
+    if (cells != (size_t)p.grid_res[0] * (size_t)p.grid_res[1] * (size_t)p.grid_res[2]) return 1;
+    if (c->fp64) splat_density(c->d, out); else splat_density(c->f, out);
+    return 0;
 }
 void oracle_wall_update(fe_wall* w, float dt) { wall_update(w, dt); }
 void oracle_set_wall(oracle_ctx* c, const fe_wall* w) {

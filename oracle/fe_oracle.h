@@ -75,6 +75,10 @@ void oracle_step(oracle_ctx* c, float max_dt);
  * max_dt (src/renderer/ui.rs:125-126, src/renderer/mod.rs:269-270); read it back (returns
  * whether one is attached). */
 void oracle_wall_update(fe_wall* w, float dt);
+/* splat_density.comp (shaders/compute/splat_density.comp:11-51) after the texture clear
+ * (src/renderer/mod.rs:386-388): the density texture of position_a, grid_res.x * grid_res.y
+ * * grid_res.z uint32 with x fastest. Returns nonzero if `cells` is not that product. */
+int oracle_splat_density(oracle_ctx* c, uint32_t* out, size_t cells);
 void oracle_set_wall(oracle_ctx* c, const fe_wall* w);
 int oracle_get_wall(oracle_ctx* c, fe_wall* out);
 void oracle_last_iterations(oracle_ctx* c, uint32_t* density, uint32_t* divergence);

@@ -59,6 +59,7 @@ def lib():
         "oracle_residual_q": ([c_float, c_float, c_float], c_uint64),
         "oracle_features": ([P, c_void_p, c_void_p], None),
         "oracle_wall_update": ([c_void_p, c_float], None),
+        "oracle_splat_density": ([P, c_void_p, c_size_t], c_int),
         "oracle_set_wall": ([P, c_void_p], None),
         "oracle_get_wall": ([P, c_void_p], c_int),
     }
@@ -202,6 +203,15 @@ class Oracle:
         w = Wall()
         on = self._l.oracle_get_wall(self._c, ctypes.addressof(w))
         return w if on else None
+
+    def splat_density(self):
+        """The density texture (grid_res z, y, x uint32) of the current position_a."""
+        p = self.params
+        rx, ry, rz = int(p.grid_res[0]), int(p.grid_res[1]), int(p.grid_res[2])
+        out = np.empty((rz, ry, rx), np.uint32)
+        if self._l.oracle_splat_density(self._c, out.ctypes.data, out.size):
+            raise ValueError("oracle splat: bad grid_res")
+        return out
 
     def read_stats(self):
         out = (c_uint32 * 3)()
