@@ -77,6 +77,7 @@ def parse():
     ap.add_argument("--batch-frames", type=int, default=10)
     ap.add_argument("--no-sort-bench", action="store_true", help="skip the isolated 2^22 sort measurement")
     ap.add_argument("--no-splat", action="store_true", help="skip the density-splat (render hand-off) measurement")
+    ap.add_argument("--no-consistent", action="store_true", help="skip the consistent divergence-order measurement")
     ap.add_argument("--learned-collect", type=int, default=100, help="frames sampled before training")
     ap.add_argument("--learned-eval", type=int, default=400, help="frames compared after training")
     ap.add_argument("--copy-probe", action="store_true",
@@ -253,6 +254,8 @@ def main():
     if not args.no_splat:
         out["render_handoff"] = splat_measurement(fe, sim, w, args.no_cpu_baseline)
     sim.close()
+    if not args.no_consistent:
+        out["consistent_divergence"] = divergence_order_measurement(fe, w, opts, args.steps, args.warmup, tuned)
     if not args.no_learned and args.solver == "converge":
         out["learned_warm_start"] = learned_comparison(fe, w, opts, args.learned_collect, args.learned_eval, tuned)
     if not args.no_batch:
@@ -294,6 +297,31 @@ def sort_measurement(fe, scene, log2n=22, warmup=3, reps=15):
                         "measures": "submit-to-fence wall clock", "source": "scripts/sort_benchmark.csv:20-21"}
     # 68 algorithmic bytes per pair for a four-pass LSD sort of 8-byte pairs, at 8 TB/s
     out["radix_hbm_floor_ms"] = round(68 * n / 8e12 * 1e3, 4)
+    return out
+
+
+def divergence_order_measurement(fe, w, opts, steps, warmup, tuned=None):
+    """The same frames (warm-up, then `steps` timed with HIP events) with the reference and the
+    consistent divergence order (FE_DIVERGENCE_CONSISTENT, a non-parity mode): steps/s and the
+    Jacobi sweeps per step."""
+    out = {"workload": w.name, "frames": f"{warmup} warm-up + {steps} timed, from rest"}
+    for name, order in (("reference", fe.FE_DIVERGENCE_REFERENCE), ("consistent", fe.FE_DIVERGENCE_CONSISTENT)):
+        o = fe.FeOptions.from_buffer_copy(opts)
+        o.divergence_order = order
+        sim = fe.Simulation(w.params, w.positions, device=0, options=o)
+        if tuned:
+            for k, v in tuned.items():
+                sim.set_tune(k, v["block"], v["items"])
+        sim.run(warmup, w.params.dt)
+        sim.reset_iteration_totals()
+        sim.timer_start()
+        sim.run(steps, w.params.dt)
+        ms = sim.timer_stop()
+        d, v, frames = sim.iteration_totals()
+        sim.close()
+        out[name] = {"steps_per_s": round(steps / (ms / 1e3), 3),
+                     "sweeps_per_step": {"density": round(d / max(frames, 1), 3),
+                                         "divergence": round(v / max(frames, 1), 3)}}
     return out
 
 

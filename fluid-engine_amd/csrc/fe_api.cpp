@@ -427,6 +427,7 @@ fe_status neighbor_search(fe_ctx* c) {
            "schedule sort");
     FE_HIP(fe::launch_schedule_mirrors(c), "schedule mirrors");
     c->ns_fresh = true;
+    c->perm_valid = true;
     c->da_fresh = false;
     c->ns_sort = c->options.sort_algorithm;
     c->ns_h = c->params.smoothing_radius;
@@ -733,6 +734,8 @@ fe_status fe_set_options(fe_ctx* c, const fe_options* o) {
         return fail(FE_ERR_INVALID_ARGUMENT, "unknown solver_mode");
     if (o->warm_start != FE_WARM_START_REFERENCE && o->warm_start != FE_WARM_START_LEARNED)
         return fail(FE_ERR_INVALID_ARGUMENT, "unknown warm_start");
+    if (o->divergence_order != FE_DIVERGENCE_REFERENCE && o->divergence_order != FE_DIVERGENCE_CONSISTENT)
+        return fail(FE_ERR_INVALID_ARGUMENT, "unknown divergence_order");
     if (o->warm_start == FE_WARM_START_LEARNED && !fe::learn_ready(c, 0) && !fe::learn_ready(c, 1))
         return fail(FE_ERR_UNSUPPORTED,
                     "warm_start LEARNED needs a network: fe_learn_train or fe_learn_set_weights first");
@@ -772,6 +775,7 @@ fe_status fe_upload(fe_ctx* c, int field, const void* src, size_t bytes) {
     DeviceGuard g(c->device);
     if (field == FE_FIELD_POSITION_B || field == FE_FIELD_GRID_ENTRIES || field == FE_FIELD_GRID_START)
         c->list_valid = false;
+    if (field == FE_FIELD_GRID_ENTRIES) c->perm_valid = false;
     c->ns_fresh = false;
     c->da_fresh = false;
     fe::learn_invalidate(c);
@@ -860,6 +864,7 @@ FE_STAGE(fe_neighbor_search, {
 FE_STAGE(fe_sort, {
     if (c->nscenes > 1) return fail(FE_ERR_UNSUPPORTED, "the stand-alone sort stage takes one scene");
     drop_graph(c);
+    c->perm_valid = false;
     if (c->options.sort_algorithm == FE_SORT_BITONIC)
         FE_HIP(fe::bitonic_sort(c), "bitonic sort");
     else
@@ -901,7 +906,11 @@ FE_STAGE(fe_divergence_source, {
     if (s != FE_OK) return s;
     FE_HIP(fe::launch_divergence_source(c), "divergence_source");
 })
-FE_STAGE(fe_divergence_integration, { FE_HIP(fe::launch_divergence_integration(c), "divergence_integration"); })
+FE_STAGE(fe_divergence_integration, {
+    if (c->options.divergence_order == FE_DIVERGENCE_CONSISTENT && !c->perm_valid)
+        return fail(FE_ERR_UNSUPPORTED, "the consistent divergence order needs a neighbour search first");
+    FE_HIP(fe::launch_divergence_integration(c), "divergence_integration");
+})
 FE_STAGE(fe_stats, { FE_HIP(fe::launch_stats(c), "stats"); })
 
 fe_status fe_solve_density(fe_ctx* c, uint32_t* iterations) {

@@ -1315,9 +1315,12 @@ hipError_t launch_density_source(fe_ctx* c) {
 }
 
 hipError_t launch_divergence_source(fe_ctx* c) {
-    hipError_t e = need(c, kMirrorPos | kMirrorVelA);
+    // The consistent order reads velocity_b, which the reorder filled in the current order,
+    // through its mirror (xyz; w holds m / rho and is not read).
+    const bool consistent = c->options.divergence_order == FE_DIVERGENCE_CONSISTENT;
+    hipError_t e = need(c, kMirrorPos | (consistent ? kMirrorVelB : kMirrorVelA));
     if (e != hipSuccess) return e;
-    DivergenceSource pol{c->va_m, c->dens, c->pres, c->src, c->pos_m, c->src_m, c->pres_m};
+    DivergenceSource pol{consistent ? c->vb_m : c->va_m, c->dens, c->pres, c->src, c->pos_m, c->src_m, c->pres_m};
     if (learn_features_wanted(c, 1)) {
         if ((e = learn_prev(c, 1)) != hipSuccess) return e;
         DivergenceSourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n};

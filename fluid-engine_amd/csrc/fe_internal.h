@@ -238,6 +238,7 @@ struct fe_ctx {
     // particles since the last neighbour search + density_alpha): the next frame-start
     // search would reproduce them bit for bit, so only velocity_b is re-gathered.
     bool ns_fresh = false;
+    bool perm_valid = false;   // vals / inv hold the permutation and schedule of the last search
     bool da_fresh = false;
     uint32_t ns_sort = 0;
     float ns_h = 0.0f;

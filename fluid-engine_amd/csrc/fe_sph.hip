@@ -179,11 +179,9 @@ __global__ __launch_bounds__(B) void k_pressure_integration(uint32_t n, Sph s0, 
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
 
 // divergence_integration.comp:14-43.
-__device__ __forceinline__ void divergence_integration_one(uint32_t i, const Sph& s0, const Scenes& sc,
-                                                           float4* __restrict__ vel, const float4* __restrict__ pacc,
-                                                           float4* __restrict__ colors) {
-    const float dt = scene_at(sc, s0, i).dt;
-    const float4 v = vel[i], a = pacc[i];
+// divergence_integration.comp: v + a dt and the speed colour ramp of one particle.
+__device__ __forceinline__ void divergence_step(float dt, const float4& v, const float4& a, float4& v_out,
+                                                float4& colour) {
     float vx = v.x + a.x * dt, vy = v.y + a.y * dt, vz = v.z + a.z * dt;
     float speed = sqrtf(vx * vx + vy * vy + vz * vz);
     float t = speed / 4.0f;
@@ -196,8 +194,17 @@ __device__ __forceinline__ void divergence_integration_one(uint32_t i, const Sph
         float u = (t - 0.5f) * 2.0f;
         c0 = mixf(0.2f, 1.0f, u); c1 = mixf(0.5f, 1.0f, u); c2 = mixf(1.0f, 1.0f, u);
     }
-    vel[i] = make_float4(vx, vy, vz, 0.0f);
-    colors[i] = make_float4(c0, c1, c2, 1.0f);
+    v_out = make_float4(vx, vy, vz, 0.0f);
+    colour = make_float4(c0, c1, c2, 1.0f);
+}
+
+__device__ __forceinline__ void divergence_integration_one(uint32_t i, const Sph& s0, const Scenes& sc,
+                                                           float4* __restrict__ vel, const float4* __restrict__ pacc,
+                                                           float4* __restrict__ colors) {
+    float4 v, col;
+    divergence_step(scene_at(sc, s0, i).dt, vel[i], pacc[i], v, col);
+    vel[i] = v;
+    colors[i] = col;
 }
 
 template <int B, int IPT>
@@ -208,6 +215,34 @@ __global__ __launch_bounds__(B) void k_divergence_integration(uint32_t n, Sph s0
     for (int k = 0; k < IPT; k++) {
         const uint32_t i = (blockIdx.x * IPT + k) * B + threadIdx.x;
         if (i < n) divergence_integration_one(i, s0, sc, vel, pacc, colors);
+    }
+}
+
+// The consistent divergence order: the velocity of the particle at sorted index i is
+// velocity_b[i]; its corrected value and colour go back to the same particle, index vals[i]
+// of velocity_a / colors, and to velocity_b[i] and the xyz of its schedule mirror (whose w
+// keeps m / rho).
+template <int B, int IPT>
+__global__ __launch_bounds__(B) void k_divergence_integration_consistent(
+    uint32_t n, Sph s0, Scenes sc, const uint32_t* __restrict__ vals, const uint32_t* __restrict__ inv,
+    float4* __restrict__ vel_a, float4* __restrict__ vel_b, float4* __restrict__ vb_m, const float4* __restrict__ pacc,
+    float4* __restrict__ colors) {
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const uint32_t i = (blockIdx.x * IPT + k) * B + threadIdx.x;
+        if (i >= n) continue;
+        float4 v, col;
+        divergence_step(scene_at(sc, s0, i).dt, vel_b[i], pacc[i], v, col);
+        const uint32_t j = vals[i];
+        vel_a[j] = v;
+        colors[j] = col;
+        vel_b[i] = v;
+        if (vb_m) {
+            float* m = reinterpret_cast<float*>(vb_m + inv[i]);
+            m[0] = v.x;
+            m[1] = v.y;
+            m[2] = v.z;
+        }
     }
 }
 
@@ -618,8 +653,17 @@ hipError_t launch_pressure_integration(fe_ctx* c) {
 hipError_t launch_divergence_integration(fe_ctx* c) {
     c->mirrors &= ~kMirrorVelA;
     Launch L(c, "divergence_integration");
-    FE_STREAM_DISPATCH(k_divergence_integration, c->tune[kTuneDivergenceIntegration], c->n, c->stream, c->n,
-                       sph_of(c), scenes_of(c), c->vel_a, c->pacc, c->colors);
+    if (c->options.divergence_order == FE_DIVERGENCE_CONSISTENT) {
+        // vals and inv come from the last neighbour search (the callers check it); the
+        // velocity_b mirror is updated in place when it is current.
+        float4* vbm = (c->mirrors & kMirrorVelB) ? c->vb_m : nullptr;
+        FE_STREAM_DISPATCH(k_divergence_integration_consistent, c->tune[kTuneDivergenceIntegration], c->n, c->stream,
+                           c->n, sph_of(c), scenes_of(c), c->vals[0], c->inv, c->vel_a, c->vel_b, vbm, c->pacc,
+                           c->colors);
+    } else {
+        FE_STREAM_DISPATCH(k_divergence_integration, c->tune[kTuneDivergenceIntegration], c->n, c->stream, c->n,
+                           sph_of(c), scenes_of(c), c->vel_a, c->pacc, c->colors);
+    }
     return hipGetLastError();
 }
 

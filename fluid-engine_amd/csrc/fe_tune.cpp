@@ -78,8 +78,10 @@ fe_status fe_autotune(fe_ctx* c, uint32_t reps) {
     FE_HIP(hipStreamSynchronize(c->stream), "tune wait");
     const size_t n = c->n;
     Snapshot snap;
-    for (auto f : std::vector<std::pair<void*, size_t>>{{c->pos_a, 16 * n}, {c->vel_a, 16 * n}, {c->colors, 16 * n},
-                                                        {c->pacc, 16 * n}, {c->pres, 4 * n}, {c->src, 4 * n}}) {
+    // velocity_b: the consistent divergence order writes it too
+    for (auto f : std::vector<std::pair<void*, size_t>>{{c->pos_a, 16 * n}, {c->vel_a, 16 * n}, {c->vel_b, 16 * n},
+                                                        {c->colors, 16 * n}, {c->pacc, 16 * n}, {c->pres, 4 * n},
+                                                        {c->src, 4 * n}}) {
         Snapshot::Item it{f.first, nullptr, f.second};
         FE_HIP(hipMalloc(&it.copy, it.bytes), "hipMalloc tune snapshot");
         snap.items.push_back(it);

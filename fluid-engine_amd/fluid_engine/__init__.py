@@ -16,7 +16,8 @@ import numpy as np
 from . import scene, tuning
 from ._ffi import (EXPORTED_SYMBOLS, FE_MLP_HIDDEN, FE_MLP_INPUTS, FE_MLP_NORM, FE_MLP_PARAMS, FE_NET_DENSITY,
                    FE_NET_DIVERGENCE, FE_SOLVER_CONVERGE, FE_SOLVER_FIXED, FE_SORT_BITONIC, FE_SORT_RADIX,
-                   FE_WARM_START_LEARNED, FE_WARM_START_REFERENCE, FIELDS, INT_FIELDS, LIB_PATH, FeOptions,
+                   FE_WARM_START_LEARNED, FE_WARM_START_REFERENCE, FE_DIVERGENCE_REFERENCE, FE_DIVERGENCE_CONSISTENT,
+                   FIELDS, INT_FIELDS, LIB_PATH, FeOptions,
                    FeParams, FeTuneEntry, FeWall, FluidEngineError, check, load_library)
 from .collision import CollisionBox
 
@@ -25,7 +26,7 @@ __all__ = [
     "default_options", "FE_SORT_BITONIC", "FE_SORT_RADIX", "FE_SOLVER_FIXED", "FE_SOLVER_CONVERGE",
     "FE_WARM_START_REFERENCE", "FE_WARM_START_LEARNED", "FIELDS", "EXPORTED_SYMBOLS", "LIB_PATH",
     "FE_NET_DENSITY", "FE_NET_DIVERGENCE", "FE_MLP_INPUTS", "FE_MLP_HIDDEN", "FE_MLP_PARAMS", "FE_MLP_NORM",
-    "Batch", "CollisionBox", "FeWall",
+    "Batch", "CollisionBox", "FeWall", "FE_DIVERGENCE_REFERENCE", "FE_DIVERGENCE_CONSISTENT",
 ]
 
 

@@ -21,6 +21,8 @@ FE_SOLVER_FIXED = 0
 FE_SOLVER_CONVERGE = 1
 FE_WARM_START_REFERENCE = 0
 FE_WARM_START_LEARNED = 1
+FE_DIVERGENCE_REFERENCE = 0
+FE_DIVERGENCE_CONSISTENT = 1
 FE_NET_DENSITY = 0
 FE_NET_DIVERGENCE = 1
 FE_MLP_INPUTS = 12
@@ -89,7 +91,8 @@ class FeOptions(Structure):
         ("divergence_tol", c_float),
         ("max_iterations", c_uint32),
         ("warm_start", c_uint32),
-        ("reserved", c_uint32 * 6),
+        ("divergence_order", c_uint32),
+        ("reserved", c_uint32 * 5),
     ]
 
 

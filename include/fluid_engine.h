@@ -79,6 +79,17 @@ typedef struct fe_params {
 #define FE_WARM_START_REFERENCE 0u
 #define FE_WARM_START_LEARNED 1u
 
+/* Velocity order of the divergence phase. REFERENCE reproduces the reference: after the
+ * post-integrate reorder, divergence_source_term.comp:47 and divergence_integration.comp:49
+ * read and correct velocity_a, still in the previous sort order, at the new order's indices,
+ * and the next substep's viscosity reads velocity_b from before the correction
+ * (src/renderer/mod.rs:289-336). CONSISTENT (no reference counterpart, not a parity mode):
+ * the divergence source uses each particle's own velocity (velocity_b, reordered with the
+ * positions), and the divergence integration writes the corrected velocity and the colour back
+ * to the same particle in velocity_a / colors and to velocity_b. */
+#define FE_DIVERGENCE_REFERENCE 0u
+#define FE_DIVERGENCE_CONSISTENT 1u
+
 /* Run options. The first five mirror the runtime flags of `AppUI`
  * (src/renderer/ui.rs:12-49): sort_algorithm, use_cfl, use_solver_error_threshold,
  * density_error_pct, divergence_error_pct, whose host logic is in Renderer::step
@@ -94,7 +105,8 @@ typedef struct fe_options {
     float divergence_tol;                /* 1e-2 (relative) for FE_SOLVER_CONVERGE */
     uint32_t max_iterations;             /* 100 */
     uint32_t warm_start;                 /* FE_WARM_START_REFERENCE */
-    uint32_t reserved[6];
+    uint32_t divergence_order;           /* FE_DIVERGENCE_REFERENCE */
+    uint32_t reserved[5];
 } fe_options;
 
 /* Device buffers of `GpuPhysicsData` (src/entities/particle.rs:137-161).

@@ -522,10 +522,14 @@ template <class R>
 void divergence_source(State<R>& s) {
     const R h = s.P(s.p.smoothing_radius), mass = s.P(s.p.particle_mass);
     const R hh = h * h;
+    // FE_DIVERGENCE_CONSISTENT (no reference counterpart): each particle's own velocity, as
+    // reordered with the positions into velocity_b; the reference reads velocity_a, still in
+    // the previous order, at the current indices.
+    const std::vector<R>& vel = s.o.divergence_order == FE_DIVERGENCE_CONSISTENT ? s.vel_b : s.vel_a;
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < (int64_t)s.n; ii++) {
         uint32_t i = (uint32_t)ii;
-        V3<R> pi = s.v3(s.pos_b, i), vi = s.v3(s.vel_a, i);
+        V3<R> pi = s.v3(s.pos_b, i), vi = s.v3(vel, i);
         R div = R(0);
         for_each_candidate(s, i, [&](uint32_t j) {
             V3<R> rv = sub(pi, s.v3(s.pos_b, j));
@@ -534,7 +538,7 @@ void divergence_source(State<R>& s) {
             R r = std::sqrt(r2);
             if (r > Lit<R>::eps6) {
                 V3<R> g = kernel_grad(rv, r, h);
-                V3<R> vd = sub(vi, s.v3(s.vel_a, j));
+                V3<R> vd = sub(vi, s.v3(vel, j));
                 div += mass * dot(vd, g);
             }
         });
@@ -553,8 +557,11 @@ void divergence_integration(State<R>& s) {
     const R deep[3] = {R(0.05), R(0.1), R(0.5)};
     const R water[3] = {R(0.2), R(0.5), R(1.0)};
     const R foam[3] = {R(1), R(1), R(1)};
+    // FE_DIVERGENCE_CONSISTENT: the particle at sorted index i is entries[i].index of
+    // velocity_a / colors; its corrected velocity also goes to velocity_b[i].
+    const bool consistent = s.o.divergence_order == FE_DIVERGENCE_CONSISTENT;
     for (uint32_t i = 0; i < s.n; i++) {
-        V3<R> v = s.v3(s.vel_a, i), a = s.v3(s.pacc, i);
+        V3<R> v = s.v3(consistent ? s.vel_b : s.vel_a, i), a = s.v3(s.pacc, i);
         V3<R> nv = add(v, scale_r(a, dt));
         R speed = std::sqrt(dot(nv, nv));
         R t = speed / R(4);
@@ -563,8 +570,10 @@ void divergence_integration(State<R>& s) {
         for (int k = 0; k < 3; k++)
             col[k] = t < R(0.5) ? mixf(deep[k], water[k], t * R(2))
                                 : mixf(water[k], foam[k], (t - R(0.5)) * R(2));
-        s.put4(s.vel_a, i, nv, R(0));
-        s.put4(s.colors, i, V3<R>{col[0], col[1], col[2]}, R(1));
+        const uint32_t dst = consistent ? s.entries[2 * (size_t)i + 1] : i;
+        s.put4(s.vel_a, dst, nv, R(0));
+        s.put4(s.colors, dst, V3<R>{col[0], col[1], col[2]}, R(1));
+        if (consistent) s.put4(s.vel_b, i, nv, R(0));
     }
 }
 
