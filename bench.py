@@ -78,6 +78,7 @@ def parse():
     ap.add_argument("--no-sort-bench", action="store_true", help="skip the isolated 2^22 sort measurement")
     ap.add_argument("--no-splat", action="store_true", help="skip the density-splat (render hand-off) measurement")
     ap.add_argument("--no-consistent", action="store_true", help="skip the consistent divergence-order measurement")
+    ap.add_argument("--no-boundary", action="store_true", help="skip the wall boundary-particles measurement")
     ap.add_argument("--learned-collect", type=int, default=100, help="frames sampled before training")
     ap.add_argument("--learned-eval", type=int, default=400, help="frames compared after training")
     ap.add_argument("--copy-probe", action="store_true",
@@ -256,6 +257,8 @@ def main():
     sim.close()
     if not args.no_consistent:
         out["consistent_divergence"] = divergence_order_measurement(fe, w, opts, args.steps, args.warmup, tuned)
+    if not args.no_boundary:
+        out["boundary_particles"] = boundary_measurement(fe, w, opts, args.steps, args.warmup)
     if not args.no_learned and args.solver == "converge":
         out["learned_warm_start"] = learned_comparison(fe, w, opts, args.learned_collect, args.learned_eval, tuned)
     if not args.no_batch:
@@ -322,6 +325,42 @@ def divergence_order_measurement(fe, w, opts, steps, warmup, tuned=None):
         out[name] = {"steps_per_s": round(steps / (ms / 1e3), 3),
                      "sweeps_per_step": {"density": round(d / max(frames, 1), 3),
                                          "divergence": round(v / max(frames, 1), 3)}}
+    return out
+
+
+def boundary_measurement(fe, w, opts, steps, warmup):
+    """The same frames with the reference's clamp-only walls and with wall boundary particles
+    (FE_BOUNDARY_PARTICLES, a non-parity mode): steps/s, Jacobi sweeps per step, and after the
+    frames the mean density of the fluid layer on the floor (particles within 2 r of it) and the
+    number of particles the floor's collision clamp holds."""
+    import numpy as np
+    out = {"workload": w.name, "frames": f"{warmup} warm-up + {steps} timed, from rest"}
+    r = float(w.params.particle_radius)
+    rho0 = float(w.params.target_density)
+    for name, b in (("clamp", fe.FE_BOUNDARY_CLAMP), ("particles", fe.FE_BOUNDARY_PARTICLES)):
+        o = fe.FeOptions.from_buffer_copy(opts)
+        o.boundary = b
+        sim = fe.Simulation(w.params, w.positions, device=0, options=o)
+        sim.run(warmup, w.params.dt)
+        sim.reset_iteration_totals()
+        sim.timer_start()
+        sim.run(steps, w.params.dt)
+        ms = sim.timer_stop()
+        d, v, frames = sim.iteration_totals()
+        pos = sim.download("position_b")
+        rho = sim.download("densities")
+        pos_a = sim.download("position_a")
+        sim.close()
+        floor = pos[:, 1] < float(w.params.box_min[1]) + 2 * r
+        # resting on the collision clamp: the clamp puts a particle at box_min.y + r + 0.001
+        on_clamp = pos_a[:, 1] <= float(w.params.box_min[1]) + r + 0.0015
+        out[name] = {"steps_per_s": round(steps / (ms / 1e3), 3),
+                     "sweeps_per_step": {"density": round(d / max(frames, 1), 3),
+                                         "divergence": round(v / max(frames, 1), 3)},
+                     "floor_layer": {"particles": int(floor.sum()),
+                                     "mean_rho_over_rho0": round(float(rho[floor].mean()) / rho0, 4)},
+                     "held_by_the_floor_clamp": int(on_clamp.sum()),
+                     "all_mean_rho_over_rho0": round(float(rho.mean()) / rho0, 4)}
     return out
 
 

@@ -69,6 +69,7 @@ void free_ctx(fe_ctx* c) {
     fe::sort_scratch_free(&c->sort);
     fe::learn_free(c);
     fe::splat_free(c);
+    if (c->walls_psi) (void)hipFree(c->walls_psi);
     for (void* p : {(void*)c->scene_tab, (void*)c->sid, (void*)c->scene_stats})
         if (p) (void)hipFree(p);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -455,8 +456,10 @@ fe_status step(fe_ctx* c, float max_dt) {
     // particles have not moved since the last search (one substep per frame: the divergence
     // phase leaves position_a alone), hash, sort, offsets, position_b, the densities and the
     // factors would come out bit-identical, so only velocity_b is gathered again.
+    // (With wall boundary particles the densities also depend on the box, which a moving wall
+    // changes between frames.)
     if (c->ns_fresh && c->da_fresh && c->ns_sort == c->options.sort_algorithm &&
-        c->ns_h == c->params.smoothing_radius) {
+        c->ns_h == c->params.smoothing_radius && fe::walls_current(c)) {
         FE_HIP(fe::launch_reorder_velocity(c), "reorder velocity");
     } else {
         if ((s = neighbor_search(c)) != FE_OK) return s;
@@ -711,6 +714,8 @@ fe_status fe_set_params(fe_ctx* c, const fe_params* p) {
     drop_graph(c);
     uint32_t fast = 0;
     FE_HIP(fe::exact_math_flags(c, *p, &fast), "exact arithmetic check");
+    // the densities of the last density_alpha used the old mass
+    if (std::memcmp(&p->particle_mass, &c->params.particle_mass, sizeof(float)) != 0) c->da_fresh = false;
     c->params = *p;
     c->exact_fast = fast;
     c->morton_bits = morton_bits_for(*p);
@@ -736,6 +741,24 @@ fe_status fe_set_options(fe_ctx* c, const fe_options* o) {
         return fail(FE_ERR_INVALID_ARGUMENT, "unknown warm_start");
     if (o->divergence_order != FE_DIVERGENCE_REFERENCE && o->divergence_order != FE_DIVERGENCE_CONSISTENT)
         return fail(FE_ERR_INVALID_ARGUMENT, "unknown divergence_order");
+    if (o->boundary != FE_BOUNDARY_CLAMP && o->boundary != FE_BOUNDARY_PARTICLES)
+        return fail(FE_ERR_INVALID_ARGUMENT, "unknown boundary");
+    if (o->boundary == FE_BOUNDARY_PARTICLES) {
+        if (c->nscenes > 1) return fail(FE_ERR_UNSUPPORTED, "boundary particles take a single-scene context");
+        const fe_params& p = c->params;
+        const float sb = 2.0f * p.particle_radius;
+        if (!(sb > 0.0f) || !std::isfinite(sb) || !(p.smoothing_radius / sb <= 16.0f))
+            return fail(FE_ERR_INVALID_ARGUMENT, "boundary particles need 0 < smoothing_radius <= 32 particle_radius");
+        double samples = 0.0;
+        for (int a = 0; a < 3; a++) {
+            const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+            const double eu = (p.box_max[u] - p.box_min[u]) / sb, ev = (p.box_max[v] - p.box_min[v]) / sb;
+            if (!(eu >= 1.0 && ev >= 1.0) || !std::isfinite(eu * ev))
+                return fail(FE_ERR_INVALID_ARGUMENT, "boundary particles need a box at least 2 r wide on each axis");
+            samples += 2.0 * eu * ev;
+        }
+        if (samples > (double)(1u << 26)) return fail(FE_ERR_INVALID_ARGUMENT, "boundary particles: too many wall samples");
+    }
     if (o->warm_start == FE_WARM_START_LEARNED && !fe::learn_ready(c, 0) && !fe::learn_ready(c, 1))
         return fail(FE_ERR_UNSUPPORTED,
                     "warm_start LEARNED needs a network: fe_learn_train or fe_learn_set_weights first");
@@ -746,6 +769,8 @@ fe_status fe_set_options(fe_ctx* c, const fe_options* o) {
         return fail(FE_ERR_UNSUPPORTED,
                     "a batch runs the radix sort, fixed dt, no early-exit flags and the reference warm start");
     drop_graph(c);
+    // the last densities were formed with (or without) the wall samples
+    if (o->boundary != c->options.boundary) c->da_fresh = false;
     c->options = *o;
     return upload_scene_tab(c);
 }

@@ -221,6 +221,57 @@ __device__ __forceinline__ float grad_of_q(float r, float q, const Sph& s) {
 }
 __device__ __forceinline__ float exact_grad(float r, const Sph& s) { return grad_of_q(r, q_big(r, s), s); }
 
+// Wall samples of FE_BOUNDARY_PARTICLES (no reference counterpart): each face f of the box
+// (f = 2 a + side: axis a, side 0 at box_min, 1 at box_max) carries a lattice of static samples
+// at spacing sb = 2 r in the plane pl[f], r outside the face (box_min - r, box_max + r: one
+// spacing from the fluid layer the collision keeps r inside), sample (k, m) at tangential
+// coordinates min_u + (k + 1/2) sb and min_v + (m + 1/2) sb (u, v the other two axes in
+// increasing order), k < n[f][0], m < n[f][1]. psi holds each sample's
+// rho0 / sum_k W(x_b - x_k) (its boundary mass), faces concatenated from off[f], m-major.
+struct Walls {
+    const float* psi;
+    float mn[3], mx[3];
+    float pl[6];
+    float sb;
+    int R;               // window half-width in samples: ceil(h / sb) + 1
+    int n[6][2];
+    uint32_t off[6];
+};
+
+// Every wall sample within the window of point p, in a fixed order (faces 0..5, then m, then
+// k), as fn(rx, ry, rz, index) with r = p - x_b. A face is visited when |p_a - plane| < h.
+// The window is a superset of the samples within h; callers test r^2 <= h^2 themselves.
+template <class F>
+__host__ __device__ __forceinline__ void for_each_wall_sample(const Walls& w, float h, float px, float py, float pz,
+                                                              F&& fn) {
+    const float pc[3] = {px, py, pz};
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+        const int a = f >> 1;
+        const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+        const float plane = w.pl[f];
+        if (!(fabsf(pc[a] - plane) < h)) continue;
+        const int nu = w.n[f][0], nv = w.n[f][1];
+        // nearest lower sample index, clamped before the conversion (a NaN coordinate gives -1)
+        const float fu = fminf(fmaxf(floorf((pc[u] - w.mn[u]) / w.sb), -1.0f), (float)nu);
+        const float fv = fminf(fmaxf(floorf((pc[v] - w.mn[v]) / w.sb), -1.0f), (float)nv);
+        const int ku = (int)fu, kv = (int)fv;
+        const int m0 = kv - w.R < 0 ? 0 : kv - w.R, m1 = kv + w.R > nv - 1 ? nv - 1 : kv + w.R;
+        const int k0 = ku - w.R < 0 ? 0 : ku - w.R, k1 = ku + w.R > nu - 1 ? nu - 1 : ku + w.R;
+        for (int m = m0; m <= m1; m++) {
+            const float cv = w.mn[v] + ((float)m + 0.5f) * w.sb;
+            for (int k = k0; k <= k1; k++) {
+                const float cu = w.mn[u] + ((float)k + 0.5f) * w.sb;
+                float r[3];
+                r[a] = pc[a] - plane;
+                r[u] = pc[u] - cu;
+                r[v] = pc[v] - cv;
+                fn(r[0], r[1], r[2], w.off[f] + (uint32_t)m * (uint32_t)nu + (uint32_t)k);
+            }
+        }
+    }
+}
+
 // p / rho^2 of a particle whose density is <= 1e-6: pressure_force skips such neighbours.
 // A signalling-NaN payload that no arithmetic result carries.
 constexpr uint32_t kNoRho = 0x7FA00001u;

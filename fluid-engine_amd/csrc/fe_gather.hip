@@ -25,6 +25,9 @@
 // Blocks are remapped so that each XCD (one L2 each) walks a contiguous range of the
 // Morton order. A particle with more than kMaxNbr pairs, or a stale list, falls back to
 // the bucket walk, which produces the same pairs in the same order.
+#include <cmath>
+#include <cstring>
+
 #include "fe_device.h"
 
 namespace fe {
@@ -513,6 +516,17 @@ __global__ __launch_bounds__(64 * kWaves) void k_window_tables(uint32_t n, const
 // pressure_update); load(jt, g) reads the rest of the neighbour's data by schedule position
 // jt, given that record g, and self(i, t, g) the particle's own.
 
+// A wall sample's term psi_b v . grad W_ib of a velocity divergence (the sample is at rest:
+// v - v_b = v), as the fluid pairs form m (v_i - v_j) . grad W_ij.
+__device__ __forceinline__ float wall_vdot(float vx, float vy, float vz, const Sph& s, float rx, float ry, float rz,
+                                           float psi) {
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float r = sqrt_grad(r2);
+    const bool ok = !(r2 > s.hh) && r > 1e-6f;
+    const float g = exact_grad(r, s);
+    return ok ? psi * (vx * (g * rx) + vy * (g * ry) + vz * (g * rz)) : 0.0f;
+}
+
 // density_and_alpha.comp:14-72 over every pair with r^2 <= h^2 (the list includes the
 // particle itself and any r <= 1e-6 pair, which add to the density only).
 struct DensityAlpha {
@@ -525,6 +539,7 @@ struct DensityAlpha {
     struct Cand {};
     struct Acc { float rho, sgs, gx, gy, gz; };
     static constexpr bool kPaGeo = false;
+    static constexpr bool kWalls = false;
     __device__ Self self(uint32_t, uint32_t, const float4&) const { return {}; }
     __device__ Cand load(uint32_t, const float4&) const { return {}; }
     __device__ Acc zero() const { return {0.f, 0.f, 0.f, 0.f, 0.f}; }
@@ -543,6 +558,22 @@ struct DensityAlpha {
         a.gy += gok ? my : 0.0f;
         a.gz += gok ? mz : 0.0f;
         a.sgs += gok ? mx * mx + my * my + mz * mz : 0.0f;
+    }
+    // A wall sample (FE_BOUNDARY_PARTICLES): psi W to the density, psi grad W to the gradient
+    // sum; the sample does not move, so it has no term of its own in the sum of squares.
+    __device__ __forceinline__ void wall_pair(Acc& a, const Self&, const Sph& s, float rx, float ry, float rz,
+                                              float psi) const {
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const bool in = !(r2 > s.hh);
+        const float r = sqrt_exact(r2);
+        const float q = q_of(r, s);
+        const float w = w_of_q(q, s);
+        a.rho += in ? psi * w : 0.0f;
+        const bool gok = in && r > 1e-6f;
+        const float g = grad_of_q(r, q, s);
+        a.gx += gok ? psi * (g * rx) : 0.0f;
+        a.gy += gok ? psi * (g * ry) : 0.0f;
+        a.gz += gok ? psi * (g * rz) : 0.0f;
     }
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         dens[i] = a.rho;
@@ -567,6 +598,7 @@ struct Viscosity {
     struct Cand { float4 v; float mrho; };
     struct Acc { float x, y, z; };
     static constexpr bool kPaGeo = false;
+    static constexpr bool kWalls = false;
     __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {vb_m[t]}; }
     __device__ Cand load(uint32_t jt, const float4&) const {
         const float4 v = vb_m[jt];
@@ -606,6 +638,7 @@ struct DensitySource {
     struct Cand { float4 v; };
     struct Acc { float div; };
     static constexpr bool kPaGeo = false;
+    static constexpr bool kWalls = false;
     __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {va_m[t]}; }
     __device__ Cand load(uint32_t jt, const float4&) const { return {va_m[jt]}; }
     __device__ Acc zero() const { return {0.f}; }
@@ -617,6 +650,10 @@ struct DensitySource {
         const float g = exact_grad(r, s);
         const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
         a.div += ok ? s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz)) : 0.0f;
+    }
+    __device__ __forceinline__ void wall_pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
+                                              float psi) const {
+        a.div += wall_vdot(si.v.x, si.v.y, si.v.z, s, rx, ry, rz, psi);
     }
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         const float rho = dens[i];
@@ -648,6 +685,7 @@ struct DivergenceSource {
     struct Cand { float4 v; };
     struct Acc { float div; };
     static constexpr bool kPaGeo = false;
+    static constexpr bool kWalls = false;
     __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {va_m[t]}; }
     __device__ Cand load(uint32_t jt, const float4&) const { return {va_m[jt]}; }
     __device__ Acc zero() const { return {0.f}; }
@@ -659,6 +697,10 @@ struct DivergenceSource {
         const float g = exact_grad(r, s);
         const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
         a.div += ok ? s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz)) : 0.0f;
+    }
+    __device__ __forceinline__ void wall_pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
+                                              float psi) const {
+        a.div += wall_vdot(si.v.x, si.v.y, si.v.z, s, rx, ry, rz, psi);
     }
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         src[i] = -a.div;
@@ -730,9 +772,15 @@ struct DensitySourceFeat {
     struct Cand { float4 v; float pp; };
     struct Acc { float div; FeatAcc f; };
     static constexpr bool kPaGeo = false;
+    static constexpr bool kWalls = false;
     __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {base.va_m[t], pprev_m[t]}; }
     __device__ Cand load(uint32_t jt, const float4&) const { return {base.va_m[jt], pprev_m[jt]}; }
     __device__ Acc zero() const { return {0.f, feat_zero()}; }
+    // wall samples enter the source term only (the features describe fluid neighbours)
+    __device__ __forceinline__ void wall_pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
+                                              float psi) const {
+        a.div += wall_vdot(si.v.x, si.v.y, si.v.z, s, rx, ry, rz, psi);
+    }
     __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
         const float r2 = rx * rx + ry * ry + rz * rz;
         if (r2 > s.hh) return;
@@ -770,9 +818,15 @@ struct DivergenceSourceFeat {
     struct Cand { float4 v; float pp; };
     struct Acc { float div; FeatAcc f; };
     static constexpr bool kPaGeo = false;
+    static constexpr bool kWalls = false;
     __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {base.va_m[t], pprev_m[t]}; }
     __device__ Cand load(uint32_t jt, const float4&) const { return {base.va_m[jt], pprev_m[jt]}; }
     __device__ Acc zero() const { return {0.f, feat_zero()}; }
+    // wall samples enter the source term only (the features describe fluid neighbours)
+    __device__ __forceinline__ void wall_pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
+                                              float psi) const {
+        a.div += wall_vdot(si.v.x, si.v.y, si.v.z, s, rx, ry, rz, psi);
+    }
     __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
         const float r2 = rx * rx + ry * ry + rz * rz;
         if (r2 > s.hh) return;
@@ -808,6 +862,7 @@ struct PressureForce {
     float2* pa2_m;
     static constexpr int kParts = 1;
     static constexpr bool kPaGeo = false;
+    static constexpr bool kWalls = false;
     struct Self { float prho, px, py, pz; };
     struct Cand { float prho; };
     struct Acc { float x, y, z; };
@@ -827,6 +882,18 @@ struct PressureForce {
         const bool ok = !(r2 > s.hh) && r > 1e-6f && __float_as_uint(c.prho) != kNoRho;
         const float g = exact_grad(r, s);   // finite, so a skipped pair adds (+0) g r = +/-0
         const float mt = ok ? s.mass * (si.prho + c.prho) : 0.0f;
+        a.x += mt * (g * rx);
+        a.y += mt * (g * ry);
+        a.z += mt * (g * rz);
+    }
+    // A wall sample mirrors the particle's pressure: psi (p_i / rho_i^2) grad W.
+    __device__ __forceinline__ void wall_pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
+                                              float psi) const {
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float r = sqrt_grad(r2);
+        const bool ok = !(r2 > s.hh) && r > 1e-6f;
+        const float g = exact_grad(r, s);
+        const float mt = ok ? psi * si.prho : 0.0f;
         a.x += mt * (g * rx);
         a.y += mt * (g * ry);
         a.z += mt * (g * rz);
@@ -856,6 +923,7 @@ struct PressureUpdate {
     float4* pos_w;
     static constexpr int kParts = 2;
     static constexpr bool kPaGeo = true;
+    static constexpr bool kWalls = false;
     struct Self { float ax, ay, az; };
     struct Cand { float ax, ay, az; };
     struct Acc { float ap; };
@@ -884,6 +952,11 @@ struct PressureUpdate {
         const float dx = si.ax - c.ax, dy = si.ay - c.ay, dz = si.az - c.az;
         a.ap += ok ? s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz)) : 0.0f;
     }
+    // A wall sample has no pressure acceleration: psi a_i . grad W.
+    __device__ __forceinline__ void wall_pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
+                                              float psi) const {
+        a.ap += wall_vdot(si.ax, si.ay, si.az, s, rx, ry, rz, psi);
+    }
     __device__ __forceinline__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         const float alpha_i = fac_m[t], rho_i = dens_m[t], src_i = src_m[t];
         float p = pres_m[t];
@@ -904,6 +977,46 @@ struct PressureUpdate {
         return q;
     }
 };
+
+// A policy with the wall samples of FE_BOUNDARY_PARTICLES: after its fluid pairs, every
+// wall sample within reach of the particle adds the policy's wall_pair term.
+template <class P>
+struct Walled : P {
+    Walls w;
+    static constexpr bool kWalls = true;
+};
+
+template <class P>
+__device__ __forceinline__ void wall_terms(const P& pol, typename P::Acc& acc, const typename P::Self& si, const Sph& s,
+                                           const float4& p) {
+    for_each_wall_sample(pol.w, s.h, p.x, p.y, p.z, [&](float rx, float ry, float rz, uint32_t b) {
+        pol.wall_pair(acc, si, s, rx, ry, rz, pol.w.psi[b]);
+    });
+}
+
+// Boundary mass of every wall sample: psi_b = rho0 / sum_k W(x_b - x_k) over the samples k
+// within h (the sample itself included), in the enumeration order of for_each_wall_sample.
+__global__ __launch_bounds__(kBlock) void k_wall_psi(Walls w, Sph s, uint32_t total, float* __restrict__ psi) {
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= total) return;
+    int f = 0;
+    while (f < 5 && b >= w.off[f + 1]) f++;
+    const int a = f >> 1;
+    const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+    const uint32_t local = b - w.off[f];
+    const int k = (int)(local % (uint32_t)w.n[f][0]), m = (int)(local / (uint32_t)w.n[f][0]);
+    float x[3];
+    x[a] = w.pl[f];
+    x[u] = w.mn[u] + ((float)k + 0.5f) * w.sb;
+    x[v] = w.mn[v] + ((float)m + 0.5f) * w.sb;
+    float sum = 0.0f;
+    for_each_wall_sample(w, s.h, x[0], x[1], x[2], [&](float rx, float ry, float rz, uint32_t) {
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float r = sqrt_exact(r2);
+        sum += !(r2 > s.hh) ? exact_w(r, s) : 0.0f;
+    });
+    psi[b] = s.rho0 / sum;
+}
 
 // The record that holds a neighbour's position: pa_m for pressure_update, else pos_m.
 template <class P>
@@ -989,6 +1102,7 @@ __device__ __forceinline__ unsigned long long gather_one(const P& pol, const Sph
             pol.pair(acc, si, s, pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, pol.load(jt, geo_of(pol, pos_m, jt)));
         });
     }
+    if constexpr (P::kWalls) wall_terms(pol, acc, si, s, pi);
     return pol.finish(i, t, acc, si, s);
 }
 
@@ -1276,9 +1390,93 @@ hipError_t launch_build_list(fe_ctx* c) {
     return e;
 }
 
+bool walls_on(const fe_ctx* c) { return c->options.boundary == FE_BOUNDARY_PARTICLES; }
+
+// The wall lattice of the current box and particle radius (psi: the context's table).
+Walls walls_of(const fe_ctx* c, uint32_t* total) {
+    Walls w{};
+    w.psi = c->walls_psi;
+    const fe_params& p = c->params;
+    for (int a = 0; a < 3; a++) {
+        w.mn[a] = p.box_min[a];
+        w.mx[a] = p.box_max[a];
+    }
+    w.sb = 2.0f * p.particle_radius;
+    w.R = (int)std::ceil(p.smoothing_radius / w.sb) + 1;
+    uint32_t off = 0;
+    for (int f = 0; f < 6; f++) {
+        const int a = f >> 1;
+        const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+        w.pl[f] = (f & 1) ? w.mx[a] + p.particle_radius : w.mn[a] - p.particle_radius;
+        const float eu = std::floor((w.mx[u] - w.mn[u]) / w.sb), ev = std::floor((w.mx[v] - w.mn[v]) / w.sb);
+        w.n[f][0] = eu > 0.0f ? (int)eu : 0;
+        w.n[f][1] = ev > 0.0f ? (int)ev : 0;
+        w.off[f] = off;
+        off += (uint32_t)w.n[f][0] * (uint32_t)w.n[f][1];
+    }
+    if (total) *total = off;
+    return w;
+}
+
+void walls_key_of(const fe_params& p, float key[9]) {
+    const float k[9] = {p.box_min[0], p.box_min[1], p.box_min[2], p.box_max[0], p.box_max[1], p.box_max[2],
+                        p.particle_radius, p.smoothing_radius, p.target_density};
+    std::memcpy(key, k, sizeof(k));
+}
+
+bool walls_current(const fe_ctx* c) {
+    if (!walls_on(c)) return true;
+    float key[9];
+    walls_key_of(c->params, key);
+    return c->walls_ready && std::memcmp(key, c->walls_key, sizeof(key)) == 0;
+}
+
+// (Re)compute the samples' boundary masses when the box, r, h or rho0 changed.
+hipError_t ensure_walls(fe_ctx* c) {
+    float key[9];
+    walls_key_of(c->params, key);
+    if (c->walls_ready && std::memcmp(key, c->walls_key, sizeof(key)) == 0) return hipSuccess;
+    uint32_t total = 0;
+    walls_of(c, &total);
+    if (total > (1u << 26)) return hipErrorInvalidValue;   // the box grew past the checked size
+    if (total > c->walls_cap) {
+        if (c->walls_psi) (void)hipFree(c->walls_psi);
+        c->walls_psi = nullptr;
+        c->walls_cap = 0;
+        hipError_t e = hipMalloc(&c->walls_psi, sizeof(float) * (total ? total : 1));
+        if (e != hipSuccess) return e;
+        c->walls_cap = total;
+    }
+    const Walls w = walls_of(c, nullptr);
+    if (total) {
+        Launch L(c, "wall_psi");
+        hipLaunchKernelGGL(k_wall_psi, grid_for(total), dim3(kBlock), 0, c->stream, w, sph_of(c), total, c->walls_psi);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    std::memcpy(c->walls_key, key, sizeof(key));
+    c->walls_ready = true;
+    return hipSuccess;
+}
+
+// A gather with the wall samples: one configuration (256 threads, one scene, the
+// per-particle lists or the bucket walk).
+template <class P>
+hipError_t run_walled(fe_ctx* c, const char* name, const P& pol, const unsigned long long* prev = nullptr,
+                      unsigned long long thr = 0, uint32_t kind = 0, unsigned long long* slot_out = nullptr) {
+    hipError_t e = ensure_walls(c);
+    if (e != hipSuccess) return e;
+    Walled<P> wp;
+    static_cast<P&>(wp) = pol;
+    wp.w = walls_of(c, nullptr);
+    Launch L(c, name);
+    launch_gather<Walled<P>, false, 256>(c, wp, prev, thr, kind, slot_out);
+    return hipGetLastError();
+}
+
 hipError_t launch_density_alpha_only(fe_ctx* c) {
     DensityAlpha pol{c->dens, c->fac, c->dens_m, c->fac_m, c->vb_m};
-    hipError_t e = run_list(c, "density_alpha", kTuneDensityAlpha, pol);
+    hipError_t e = walls_on(c) ? run_walled(c, "density_alpha", pol) : run_list(c, "density_alpha", kTuneDensityAlpha, pol);
     c->mirrors = (c->mirrors | kMirrorDens | kMirrorFac | kMirrorMrho) & ~kMirrorPrho;   // p / rho^2 used the old densities
     c->da_fresh = c->ns_fresh;
     return e;
@@ -1305,10 +1503,11 @@ hipError_t launch_density_source(fe_ctx* c) {
     if (learn_features_wanted(c, 0)) {
         if ((e = learn_prev(c, 0)) != hipSuccess) return e;
         DensitySourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n};
-        e = run_list(c, "density_source_feat", kTuneDensitySource, fp);
+        e = walls_on(c) ? run_walled(c, "density_source_feat", fp)
+                        : run_list(c, "density_source_feat", kTuneDensitySource, fp);
         learn_features_done(c, 0);
     } else {
-        e = run_list(c, "density_source", kTuneDensitySource, pol);
+        e = walls_on(c) ? run_walled(c, "density_source", pol) : run_list(c, "density_source", kTuneDensitySource, pol);
     }
     c->mirrors |= kMirrorPrho | kMirrorSrc | kMirrorPres;
     return e;
@@ -1324,10 +1523,12 @@ hipError_t launch_divergence_source(fe_ctx* c) {
     if (learn_features_wanted(c, 1)) {
         if ((e = learn_prev(c, 1)) != hipSuccess) return e;
         DivergenceSourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n};
-        e = run_list(c, "divergence_source_feat", kTuneDivergenceSource, fp);
+        e = walls_on(c) ? run_walled(c, "divergence_source_feat", fp)
+                        : run_list(c, "divergence_source_feat", kTuneDivergenceSource, fp);
         learn_features_done(c, 1);
     } else {
-        e = run_list(c, "divergence_source", kTuneDivergenceSource, pol);
+        e = walls_on(c) ? run_walled(c, "divergence_source", pol)
+                        : run_list(c, "divergence_source", kTuneDivergenceSource, pol);
     }
     c->mirrors |= kMirrorPrho | kMirrorSrc | kMirrorPres;
     return e;
@@ -1337,7 +1538,8 @@ hipError_t launch_pressure_force(fe_ctx* c, const unsigned long long* prev, unsi
     hipError_t e = need(c, kMirrorPos | kMirrorPrho);
     if (e != hipSuccess) return e;
     PressureForce pol{c->defer_sweep_writes ? nullptr : c->pacc, c->pa_m, c->pa2_m};
-    e = run_sweep(c, "pressure_force", kTunePressureForce, pol, prev, thr, kind, nullptr);
+    e = walls_on(c) ? run_walled(c, "pressure_force", pol, prev, thr, kind, nullptr)
+                    : run_sweep(c, "pressure_force", kTunePressureForce, pol, prev, thr, kind, nullptr);
     c->mirrors |= kMirrorPacc;
     c->pending_pacc |= c->defer_sweep_writes;
     return e;
@@ -1349,7 +1551,8 @@ hipError_t launch_pressure_update(fe_ctx* c, unsigned long long* out, const unsi
     if (e != hipSuccess) return e;
     PressureUpdate pol{c->pa_m,   c->pa2_m,  c->fac_m, c->src_m, c->dens_m, c->defer_sweep_writes ? nullptr : c->pres,
                        c->pres_m, c->pos_m};
-    e = run_sweep(c, "pressure_update", kTunePressureUpdate, pol, prev, thr, kind, out);
+    e = walls_on(c) ? run_walled(c, "pressure_update", pol, prev, thr, kind, out)
+                    : run_sweep(c, "pressure_update", kTunePressureUpdate, pol, prev, thr, kind, out);
     c->pending_pres |= c->defer_sweep_writes;
     return e;
 }

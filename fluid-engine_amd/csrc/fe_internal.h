@@ -174,6 +174,13 @@ struct fe_ctx {
 
     fe::SplatState splat;
 
+    // FE_BOUNDARY_PARTICLES: the wall samples' boundary masses and the geometry they were
+    // computed for (recomputed when the box, r, h or rho0 change).
+    float* walls_psi = nullptr;
+    size_t walls_cap = 0;
+    float walls_key[9] = {};
+    bool walls_ready = false;
+
     // GpuPhysicsData fields
     float4* pos_a = nullptr;
     float4* pos_b = nullptr;
@@ -363,6 +370,11 @@ void learn_features_done(fe_ctx* c, int kind);
 // the summed squared error into loss_out (if not null); applies Adam when apply != 0.
 hipError_t learn_step(fe_ctx* c, int which, const float* x, const float* t, uint32_t batch, uint32_t sample_mod,
                       uint32_t seed, float lr, uint32_t apply, float* grad_out, float* loss_out);
+
+// ---- wall boundary particles (fe_gather.hip) ----
+// Whether the wall samples (when FE_BOUNDARY_PARTICLES is on) were computed for the current
+// box, r, h and rho0: the densities of the last density_alpha depend on them.
+bool walls_current(const fe_ctx* c);
 
 // ---- launchers (fe_splat.hip) ----
 // Clear and rebuild the density texture from position_a with the current box and grid_res.

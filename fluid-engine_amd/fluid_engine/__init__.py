@@ -17,6 +17,7 @@ from . import scene, tuning
 from ._ffi import (EXPORTED_SYMBOLS, FE_MLP_HIDDEN, FE_MLP_INPUTS, FE_MLP_NORM, FE_MLP_PARAMS, FE_NET_DENSITY,
                    FE_NET_DIVERGENCE, FE_SOLVER_CONVERGE, FE_SOLVER_FIXED, FE_SORT_BITONIC, FE_SORT_RADIX,
                    FE_WARM_START_LEARNED, FE_WARM_START_REFERENCE, FE_DIVERGENCE_REFERENCE, FE_DIVERGENCE_CONSISTENT,
+                   FE_BOUNDARY_CLAMP, FE_BOUNDARY_PARTICLES,
                    FIELDS, INT_FIELDS, LIB_PATH, FeOptions,
                    FeParams, FeTuneEntry, FeWall, FluidEngineError, check, load_library)
 from .collision import CollisionBox
@@ -27,6 +28,7 @@ __all__ = [
     "FE_WARM_START_REFERENCE", "FE_WARM_START_LEARNED", "FIELDS", "EXPORTED_SYMBOLS", "LIB_PATH",
     "FE_NET_DENSITY", "FE_NET_DIVERGENCE", "FE_MLP_INPUTS", "FE_MLP_HIDDEN", "FE_MLP_PARAMS", "FE_MLP_NORM",
     "Batch", "CollisionBox", "FeWall", "FE_DIVERGENCE_REFERENCE", "FE_DIVERGENCE_CONSISTENT",
+    "FE_BOUNDARY_CLAMP", "FE_BOUNDARY_PARTICLES",
 ]
 
 

@@ -23,6 +23,8 @@ FE_WARM_START_REFERENCE = 0
 FE_WARM_START_LEARNED = 1
 FE_DIVERGENCE_REFERENCE = 0
 FE_DIVERGENCE_CONSISTENT = 1
+FE_BOUNDARY_CLAMP = 0
+FE_BOUNDARY_PARTICLES = 1
 FE_NET_DENSITY = 0
 FE_NET_DIVERGENCE = 1
 FE_MLP_INPUTS = 12
@@ -92,7 +94,8 @@ class FeOptions(Structure):
         ("max_iterations", c_uint32),
         ("warm_start", c_uint32),
         ("divergence_order", c_uint32),
-        ("reserved", c_uint32 * 5),
+        ("boundary", c_uint32),
+        ("reserved", c_uint32 * 4),
     ]
 
 

@@ -90,6 +90,18 @@ typedef struct fe_params {
 #define FE_DIVERGENCE_REFERENCE 0u
 #define FE_DIVERGENCE_CONSISTENT 1u
 
+/* Boundary handling. CLAMP is the reference's: the box acts only through the collision clamp of
+ * pressure_integration.comp:36-73. PARTICLES (no reference counterpart, not a parity mode) adds
+ * static boundary samples on the six box faces (Akinci et al. 2012, as in DFSPH): a square
+ * lattice of spacing 2 particle_radius per face, sample b with boundary mass
+ * psi_b = rho0 / sum_k W(x_b - x_k) over the samples. A fluid particle within h of a face adds
+ * psi_b W to its density, psi_b grad W to the gradient sum of alpha, psi_b v_i . grad W to both
+ * source terms' velocity divergence, psi_b (p_i / rho_i^2) grad W to its pressure acceleration
+ * and psi_b a_i . grad W to A p (samples at rest, pressure mirrored). The clamp stays as a
+ * guard. Single-scene contexts; the Jacobi sweeps use the per-particle lists. */
+#define FE_BOUNDARY_CLAMP 0u
+#define FE_BOUNDARY_PARTICLES 1u
+
 /* Run options. The first five mirror the runtime flags of `AppUI`
  * (src/renderer/ui.rs:12-49): sort_algorithm, use_cfl, use_solver_error_threshold,
  * density_error_pct, divergence_error_pct, whose host logic is in Renderer::step
@@ -106,7 +118,8 @@ typedef struct fe_options {
     uint32_t max_iterations;             /* 100 */
     uint32_t warm_start;                 /* FE_WARM_START_REFERENCE */
     uint32_t divergence_order;           /* FE_DIVERGENCE_REFERENCE */
-    uint32_t reserved[5];
+    uint32_t boundary;                   /* FE_BOUNDARY_CLAMP */
+    uint32_t reserved[4];
 } fe_options;
 
 /* Device buffers of `GpuPhysicsData` (src/entities/particle.rs:137-161).

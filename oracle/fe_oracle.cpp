@@ -117,6 +117,9 @@ struct State {
     bool have_stats = false;
     fe_wall wall{};
     bool wall_on = false;
+    std::vector<R> wall_psi;      // FE_BOUNDARY_PARTICLES: boundary mass of each wall sample
+    float wall_key[9] = {};
+    bool wall_ready = false;
 
     void alloc(uint32_t n_) {
         n = n_;
@@ -308,10 +311,121 @@ void neighbor_search(State<R>& s) {
 // Solver kernels.
 
 // density_and_alpha.comp:14-72.
+// FE_BOUNDARY_PARTICLES (no reference counterpart): the wall samples of the six box faces.
+// Face f = 2 a + side (axis a, side 0 at box_min, 1 at box_max) holds a lattice of spacing
+// sb = 2 r in the plane box_min - r (side 0) or box_max + r (side 1), sample (k, m) at
+// tangential coordinates min_u + (k + 1/2) sb, min_v + (m + 1/2) sb (u, v the other two axes in
+// increasing order), k < n[f][0], m < n[f][1]; sample indices run face by face, m-major.
+struct WallGeom {
+    float mn[3], mx[3], pl[6], sb;
+    int R;
+    int n[6][2];
+    uint32_t off[6], total;
+};
+
+inline WallGeom wall_geom(const fe_params& p) {
+    WallGeom w{};
+    for (int a = 0; a < 3; a++) {
+        w.mn[a] = p.box_min[a];
+        w.mx[a] = p.box_max[a];
+    }
+    w.sb = 2.0f * p.particle_radius;
+    w.R = (int)std::ceil(p.smoothing_radius / w.sb) + 1;
+    uint32_t off = 0;
+    for (int f = 0; f < 6; f++) {
+        const int a = f >> 1, u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+        w.pl[f] = (f & 1) ? w.mx[a] + p.particle_radius : w.mn[a] - p.particle_radius;
+        const float eu = std::floor((w.mx[u] - w.mn[u]) / w.sb), ev = std::floor((w.mx[v] - w.mn[v]) / w.sb);
+        w.n[f][0] = eu > 0.0f ? (int)eu : 0;
+        w.n[f][1] = ev > 0.0f ? (int)ev : 0;
+        w.off[f] = off;
+        off += (uint32_t)w.n[f][0] * (uint32_t)w.n[f][1];
+    }
+    w.total = off;
+    return w;
+}
+
+// Every sample in the window around p: faces whose plane is closer than h, then the samples
+// within R lattice steps of p's own lattice cell, m outer, k inner; fn(r = p - x_b, index).
+template <class R, class F>
+void for_each_wall_sample(const WallGeom& w, R h, const V3<R>& p, F&& fn) {
+    const R pc[3] = {p.x, p.y, p.z};
+    for (int f = 0; f < 6; f++) {
+        const int a = f >> 1, u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+        const R plane = R(w.pl[f]);
+        if (!(std::fabs(pc[a] - plane) < h)) continue;
+        const int nu = w.n[f][0], nv = w.n[f][1];
+        const R fu = std::fmin(std::fmax(std::floor((pc[u] - R(w.mn[u])) / R(w.sb)), R(-1)), R(nu));
+        const R fv = std::fmin(std::fmax(std::floor((pc[v] - R(w.mn[v])) / R(w.sb)), R(-1)), R(nv));
+        const int ku = (int)fu, kv = (int)fv;
+        const int m0 = std::max(kv - w.R, 0), m1 = std::min(kv + w.R, nv - 1);
+        const int k0 = std::max(ku - w.R, 0), k1 = std::min(ku + w.R, nu - 1);
+        for (int m = m0; m <= m1; m++) {
+            const R cv = R(w.mn[v]) + (R(m) + R(0.5)) * R(w.sb);
+            for (int k = k0; k <= k1; k++) {
+                const R cu = R(w.mn[u]) + (R(k) + R(0.5)) * R(w.sb);
+                R r[3];
+                r[a] = pc[a] - plane;
+                r[u] = pc[u] - cu;
+                r[v] = pc[v] - cv;
+                fn(V3<R>{r[0], r[1], r[2]}, w.off[f] + (uint32_t)m * (uint32_t)nu + (uint32_t)k);
+            }
+        }
+    }
+}
+
+// Each sample's boundary mass rho0 / sum_k W(x_b - x_k) (the sample itself included), for
+// the current box, r, h and rho0; recomputed when one of them changes.
+template <class R>
+const WallGeom* walls(State<R>& s) {
+    if (s.o.boundary != FE_BOUNDARY_PARTICLES) return nullptr;
+    static thread_local WallGeom geom;
+    geom = wall_geom(s.p);
+    const float key[9] = {s.p.box_min[0], s.p.box_min[1], s.p.box_min[2], s.p.box_max[0], s.p.box_max[1],
+                          s.p.box_max[2], s.p.particle_radius, s.p.smoothing_radius, s.p.target_density};
+    if (s.wall_ready && std::memcmp(key, s.wall_key, sizeof(key)) == 0) return &geom;
+    const R h = s.P(s.p.smoothing_radius), hh = h * h, rho0 = s.P(s.p.target_density);
+    s.wall_psi.assign(geom.total, R(0));
+    for (int f = 0; f < 6; f++) {
+        const int a = f >> 1, u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+        for (int m = 0; m < geom.n[f][1]; m++)
+            for (int k = 0; k < geom.n[f][0]; k++) {
+                R x[3];
+                x[a] = R(geom.pl[f]);
+                x[u] = R(geom.mn[u]) + (R(k) + R(0.5)) * R(geom.sb);
+                x[v] = R(geom.mn[v]) + (R(m) + R(0.5)) * R(geom.sb);
+                R sum = R(0);
+                for_each_wall_sample(geom, h, V3<R>{x[0], x[1], x[2]}, [&](const V3<R>& rv, uint32_t) {
+                    const R r2 = dot(rv, rv);
+                    if (r2 > hh) return;
+                    sum += kernel_w(std::sqrt(r2), h);
+                });
+                s.wall_psi[geom.off[f] + (uint32_t)m * (uint32_t)geom.n[f][0] + (uint32_t)k] = rho0 / sum;
+            }
+    }
+    std::memcpy(s.wall_key, key, sizeof(key));
+    s.wall_ready = true;
+    return &geom;
+}
+
+// The samples' terms of a velocity divergence, psi_b (v - 0) . grad W_ib (samples at rest),
+// added one by one to the particle's running sum `acc` after its fluid pairs.
+template <class R>
+void wall_vdot(const State<R>& s, const WallGeom& w, R h, const V3<R>& p, const V3<R>& v, R& acc) {
+    const R hh = h * h;
+    for_each_wall_sample(w, h, p, [&](const V3<R>& rv, uint32_t b) {
+        R r2 = dot(rv, rv);
+        if (r2 > hh) return;
+        R r = std::sqrt(r2);
+        if (r > Lit<R>::eps6) acc += s.wall_psi[b] * dot(v, kernel_grad(rv, r, h));
+    });
+}
+
 template <class R>
 void density_alpha(State<R>& s) {
     const R h = s.P(s.p.smoothing_radius), mass = s.P(s.p.particle_mass);
     const R hh = h * h;
+    const WallGeom* wg = walls(s);
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < (int64_t)s.n; ii++) {
         uint32_t i = (uint32_t)ii;
@@ -331,6 +445,14 @@ void density_alpha(State<R>& s) {
                 grad_sum = add(grad_sum, mg);
                 sum_grad_sq += dot(mg, mg);
             }
+        });
+        if (wg) for_each_wall_sample(*wg, h, pi, [&](const V3<R>& rv, uint32_t b) {
+            // a sample: psi W to the density, psi grad W to the gradient sum (it does not move)
+            R r2 = dot(rv, rv);
+            if (r2 > hh) return;
+            R r = std::sqrt(r2);
+            density += s.wall_psi[b] * kernel_w(r, h);
+            if (r > Lit<R>::eps6) grad_sum = add(grad_sum, scale(s.wall_psi[b], kernel_grad(rv, r, h)));
         });
         s.dens[i] = density;
         R alpha = sum_grad_sq + dot(grad_sum, grad_sum);
@@ -376,6 +498,7 @@ template <class R>
 void density_source(State<R>& s) {
     const R h = s.P(s.p.smoothing_radius), mass = s.P(s.p.particle_mass);
     const R hh = h * h, dt = s.P(s.p.dt), rho0 = s.P(s.p.target_density);
+    const WallGeom* wg = walls(s);
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < (int64_t)s.n; ii++) {
         uint32_t i = (uint32_t)ii;
@@ -393,6 +516,7 @@ void density_source(State<R>& s) {
                 div += mass * dot(vd, g);
             }
         });
+        if (wg) wall_vdot(s, *wg, h, pi, vi, div);
         R source = R(0);
         if (dt > Lit<R>::eps6) source = ((rho0 - rho_i) / dt) - div;
         s.src[i] = source;
@@ -405,6 +529,7 @@ template <class R>
 void pressure_force(State<R>& s) {
     const R h = s.P(s.p.smoothing_radius), mass = s.P(s.p.particle_mass);
     const R hh = h * h;
+    const WallGeom* wg = walls(s);
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < (int64_t)s.n; ii++) {
         uint32_t i = (uint32_t)ii;
@@ -428,6 +553,13 @@ void pressure_force(State<R>& s) {
                 }
             }
         });
+        if (wg) for_each_wall_sample(*wg, h, pi, [&](const V3<R>& rv, uint32_t b) {
+            // a sample mirrors the particle's pressure: psi (p_i / rho_i^2) grad W
+            R r2 = dot(rv, rv);
+            if (r2 > hh) return;
+            R r = std::sqrt(r2);
+            if (r > Lit<R>::eps6) acc = add(acc, scale(s.wall_psi[b] * p_rho_i, kernel_grad(rv, r, h)));
+        });
         s.put4(s.pacc, i, V3<R>{-acc.x, -acc.y, -acc.z}, R(0));
     }
 }
@@ -446,6 +578,7 @@ template <class R>
 uint64_t pressure_update(State<R>& s) {
     const R h = s.P(s.p.smoothing_radius), mass = s.P(s.p.particle_mass);
     const R hh = h * h, dt = s.P(s.p.dt), omega = s.P(s.p.relax_factor);
+    const WallGeom* wg = walls(s);
     uint64_t resid = 0;
 #pragma omp parallel for schedule(static) reduction(+ : resid)
     for (int64_t ii = 0; ii < (int64_t)s.n; ii++) {
@@ -464,6 +597,7 @@ uint64_t pressure_update(State<R>& s) {
                 sum_ap += mass * dot(ad, g);
             }
         });
+        if (wg) wall_vdot(s, *wg, h, pi, ai, sum_ap);
         R ap = dt * sum_ap;
         R err = src_i - ap;
         resid += residual_q((float)err, (float)dt, s.p.target_density);
@@ -522,6 +656,7 @@ template <class R>
 void divergence_source(State<R>& s) {
     const R h = s.P(s.p.smoothing_radius), mass = s.P(s.p.particle_mass);
     const R hh = h * h;
+    const WallGeom* wg = walls(s);
     // FE_DIVERGENCE_CONSISTENT (no reference counterpart): each particle's own velocity, as
     // reordered with the positions into velocity_b; the reference reads velocity_a, still in
     // the previous order, at the current indices.
@@ -542,6 +677,7 @@ void divergence_source(State<R>& s) {
                 div += mass * dot(vd, g);
             }
         });
+        if (wg) wall_vdot(s, *wg, h, pi, vi, div);
         s.src[i] = -div;
     }
 }
@@ -969,6 +1105,18 @@ uint32_t oracle_solve_divergence(oracle_ctx* c) {
 }
 void oracle_step(oracle_ctx* c, float max_dt) {
     if (c->fp64) step(c->d, max_dt); else step(c->f, max_dt);
+}
+size_t oracle_wall_psi(oracle_ctx* c, float* out, size_t cap) {
+    if (c->fp64) {
+        const WallGeom* w = walls(c->d);
+        if (!w) return 0;
+        for (size_t k = 0; k < c->d.wall_psi.size() && k < cap; k++) out[k] = (float)c->d.wall_psi[k];
+        return c->d.wall_psi.size();
+    }
+    const WallGeom* w = walls(c->f);
+    if (!w) return 0;
+    for (size_t k = 0; k < c->f.wall_psi.size() && k < cap; k++) out[k] = c->f.wall_psi[k];
+    return c->f.wall_psi.size();
 }
 int oracle_splat_density(oracle_ctx* c, uint32_t* out, size_t cells) {
     const fe_params& p = c->fp64 ? c->d.p : c->f.p;

@@ -79,6 +79,10 @@ void oracle_wall_update(fe_wall* w, float dt);
  * (src/renderer/mod.rs:386-388): the density texture of position_a, grid_res.x * grid_res.y
  * * grid_res.z uint32 with x fastest. Returns nonzero if `cells` is not that product. */
 int oracle_splat_density(oracle_ctx* c, uint32_t* out, size_t cells);
+/* FE_BOUNDARY_PARTICLES: the wall samples' boundary masses for the current box (faces 0..5 =
+ * -x, +x, -y, +y, -z, +z, each m-major), up to cap into out; returns the sample count (0 when
+ * the option is off). */
+size_t oracle_wall_psi(oracle_ctx* c, float* out, size_t cap);
 void oracle_set_wall(oracle_ctx* c, const fe_wall* w);
 int oracle_get_wall(oracle_ctx* c, fe_wall* out);
 void oracle_last_iterations(oracle_ctx* c, uint32_t* density, uint32_t* divergence);

@@ -60,6 +60,7 @@ def lib():
         "oracle_features": ([P, c_void_p, c_void_p], None),
         "oracle_wall_update": ([c_void_p, c_float], None),
         "oracle_splat_density": ([P, c_void_p, c_size_t], c_int),
+        "oracle_wall_psi": ([P, c_void_p, c_size_t], c_size_t),
         "oracle_set_wall": ([P, c_void_p], None),
         "oracle_get_wall": ([P, c_void_p], c_int),
     }
@@ -203,6 +204,14 @@ class Oracle:
         w = Wall()
         on = self._l.oracle_get_wall(self._c, ctypes.addressof(w))
         return w if on else None
+
+    def wall_psi(self):
+        """Boundary masses of the wall samples (empty unless options.boundary is PARTICLES)."""
+        n = self._l.oracle_wall_psi(self._c, None, 0)
+        out = np.empty(n, np.float32)
+        if n:
+            self._l.oracle_wall_psi(self._c, out.ctypes.data, n)
+        return out
 
     def splat_density(self):
         """The density texture (grid_res z, y, x uint32) of the current position_a."""
