@@ -387,7 +387,9 @@ fe_status jacobi(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t*
             checked_chunks = chunks.size() - 1;
         }
     }
-    if (st == FE_OK && !chunks.empty()) {
+    // Once a scanned chunk showed convergence the sweeps still queued skip themselves on the
+    // device, so the host returns without waiting for them and the next stages queue behind.
+    if (st == FE_OK && !chunks.empty() && pending) {
         hipError_t e = hipEventSynchronize(chunks.back().ev);
         if (e != hipSuccess) st = hip_fail(e, "residual wait");
         for (size_t j = checked_chunks; st == FE_OK && pending && j < chunks.size(); j++) scan(chunks[j]);
