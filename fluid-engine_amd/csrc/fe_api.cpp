@@ -70,6 +70,7 @@ void free_ctx(fe_ctx* c) {
     fe::learn_free(c);
     fe::splat_free(c);
     if (c->walls_psi) (void)hipFree(c->walls_psi);
+    if (c->wall_g) (void)hipFree(c->wall_g);
     for (void* p : {(void*)c->scene_tab, (void*)c->sid, (void*)c->scene_stats})
         if (p) (void)hipFree(p);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -431,6 +432,7 @@ fe_status neighbor_search(fe_ctx* c) {
     FE_HIP(fe::launch_schedule_mirrors(c), "schedule mirrors");
     c->ns_fresh = true;
     c->perm_valid = true;
+    c->wall_g_fresh = false;
     c->da_fresh = false;
     c->ns_sort = c->options.sort_algorithm;
     c->ns_h = c->params.smoothing_radius;
@@ -803,6 +805,7 @@ fe_status fe_upload(fe_ctx* c, int field, const void* src, size_t bytes) {
     if (field == FE_FIELD_POSITION_B || field == FE_FIELD_GRID_ENTRIES || field == FE_FIELD_GRID_START)
         c->list_valid = false;
     if (field == FE_FIELD_GRID_ENTRIES) c->perm_valid = false;
+    if (field == FE_FIELD_POSITION_B) c->wall_g_fresh = false;
     c->ns_fresh = false;
     c->da_fresh = false;
     fe::learn_invalidate(c);

@@ -516,15 +516,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_window_tables(uint32_t n, const
 // pressure_update); load(jt, g) reads the rest of the neighbour's data by schedule position
 // jt, given that record g, and self(i, t, g) the particle's own.
 
-// A wall sample's term psi_b v . grad W_ib of a velocity divergence (the sample is at rest:
-// v - v_b = v), as the fluid pairs form m (v_i - v_j) . grad W_ij.
-__device__ __forceinline__ float wall_vdot(float vx, float vy, float vz, const Sph& s, float rx, float ry, float rz,
-                                           float psi) {
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    const float r = sqrt_grad(r2);
-    const bool ok = !(r2 > s.hh) && r > 1e-6f;
-    const float g = exact_grad(r, s);
-    return ok ? psi * (vx * (g * rx) + vy * (g * ry) + vz * (g * rz)) : 0.0f;
+// The wall samples (FE_BOUNDARY_PARTICLES) enter every gather through two per-particle sums
+// over the samples, fixed while the positions are: G = sum psi_b grad W_ib and
+// rho_w = sum psi_b W_ib (wall_g = (G, rho_w), k_wall_terms). Because the samples are at
+// rest, their velocity-divergence terms are v . G, their A p term a . G and, mirroring the
+// pressure, their pressure acceleration (p / rho^2) G; each policy's wall_add adds that once
+// after the fluid pairs.
+__device__ __forceinline__ float dot_g(float x, float y, float z, const float4& g) {
+    return x * g.x + y * g.y + z * g.z;
 }
 
 // density_and_alpha.comp:14-72 over every pair with r^2 <= h^2 (the list includes the
@@ -559,21 +558,12 @@ struct DensityAlpha {
         a.gz += gok ? mz : 0.0f;
         a.sgs += gok ? mx * mx + my * my + mz * mz : 0.0f;
     }
-    // A wall sample (FE_BOUNDARY_PARTICLES): psi W to the density, psi grad W to the gradient
-    // sum; the sample does not move, so it has no term of its own in the sum of squares.
-    __device__ __forceinline__ void wall_pair(Acc& a, const Self&, const Sph& s, float rx, float ry, float rz,
-                                              float psi) const {
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const bool in = !(r2 > s.hh);
-        const float r = sqrt_exact(r2);
-        const float q = q_of(r, s);
-        const float w = w_of_q(q, s);
-        a.rho += in ? psi * w : 0.0f;
-        const bool gok = in && r > 1e-6f;
-        const float g = grad_of_q(r, q, s);
-        a.gx += gok ? psi * (g * rx) : 0.0f;
-        a.gy += gok ? psi * (g * ry) : 0.0f;
-        a.gz += gok ? psi * (g * rz) : 0.0f;
+    // the samples' density and gradient sum (they do not move: no term in the sum of squares)
+    __device__ __forceinline__ void wall_add(Acc& a, const Self&, const Sph&, const float4& g) const {
+        a.rho += g.w;
+        a.gx += g.x;
+        a.gy += g.y;
+        a.gz += g.z;
     }
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         dens[i] = a.rho;
@@ -651,9 +641,8 @@ struct DensitySource {
         const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
         a.div += ok ? s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz)) : 0.0f;
     }
-    __device__ __forceinline__ void wall_pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
-                                              float psi) const {
-        a.div += wall_vdot(si.v.x, si.v.y, si.v.z, s, rx, ry, rz, psi);
+    __device__ __forceinline__ void wall_add(Acc& a, const Self& si, const Sph&, const float4& g) const {
+        a.div += dot_g(si.v.x, si.v.y, si.v.z, g);
     }
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         const float rho = dens[i];
@@ -698,9 +687,8 @@ struct DivergenceSource {
         const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
         a.div += ok ? s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz)) : 0.0f;
     }
-    __device__ __forceinline__ void wall_pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
-                                              float psi) const {
-        a.div += wall_vdot(si.v.x, si.v.y, si.v.z, s, rx, ry, rz, psi);
+    __device__ __forceinline__ void wall_add(Acc& a, const Self& si, const Sph&, const float4& g) const {
+        a.div += dot_g(si.v.x, si.v.y, si.v.z, g);
     }
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         src[i] = -a.div;
@@ -777,9 +765,8 @@ struct DensitySourceFeat {
     __device__ Cand load(uint32_t jt, const float4&) const { return {base.va_m[jt], pprev_m[jt]}; }
     __device__ Acc zero() const { return {0.f, feat_zero()}; }
     // wall samples enter the source term only (the features describe fluid neighbours)
-    __device__ __forceinline__ void wall_pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
-                                              float psi) const {
-        a.div += wall_vdot(si.v.x, si.v.y, si.v.z, s, rx, ry, rz, psi);
+    __device__ __forceinline__ void wall_add(Acc& a, const Self& si, const Sph&, const float4& g) const {
+        a.div += dot_g(si.v.x, si.v.y, si.v.z, g);
     }
     __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
         const float r2 = rx * rx + ry * ry + rz * rz;
@@ -823,9 +810,8 @@ struct DivergenceSourceFeat {
     __device__ Cand load(uint32_t jt, const float4&) const { return {base.va_m[jt], pprev_m[jt]}; }
     __device__ Acc zero() const { return {0.f, feat_zero()}; }
     // wall samples enter the source term only (the features describe fluid neighbours)
-    __device__ __forceinline__ void wall_pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
-                                              float psi) const {
-        a.div += wall_vdot(si.v.x, si.v.y, si.v.z, s, rx, ry, rz, psi);
+    __device__ __forceinline__ void wall_add(Acc& a, const Self& si, const Sph&, const float4& g) const {
+        a.div += dot_g(si.v.x, si.v.y, si.v.z, g);
     }
     __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
         const float r2 = rx * rx + ry * ry + rz * rz;
@@ -886,17 +872,11 @@ struct PressureForce {
         a.y += mt * (g * ry);
         a.z += mt * (g * rz);
     }
-    // A wall sample mirrors the particle's pressure: psi (p_i / rho_i^2) grad W.
-    __device__ __forceinline__ void wall_pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
-                                              float psi) const {
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const float r = sqrt_grad(r2);
-        const bool ok = !(r2 > s.hh) && r > 1e-6f;
-        const float g = exact_grad(r, s);
-        const float mt = ok ? psi * si.prho : 0.0f;
-        a.x += mt * (g * rx);
-        a.y += mt * (g * ry);
-        a.z += mt * (g * rz);
+    // the samples mirror the particle's pressure: (p_i / rho_i^2) G
+    __device__ __forceinline__ void wall_add(Acc& a, const Self& si, const Sph&, const float4& g) const {
+        a.x += si.prho * g.x;
+        a.y += si.prho * g.y;
+        a.z += si.prho * g.z;
     }
     __device__ __forceinline__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si, const Sph& s) const {
         const float4 v = make_float4(-a.x, -a.y, -a.z, 0.0f);
@@ -952,10 +932,9 @@ struct PressureUpdate {
         const float dx = si.ax - c.ax, dy = si.ay - c.ay, dz = si.az - c.az;
         a.ap += ok ? s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz)) : 0.0f;
     }
-    // A wall sample has no pressure acceleration: psi a_i . grad W.
-    __device__ __forceinline__ void wall_pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
-                                              float psi) const {
-        a.ap += wall_vdot(si.ax, si.ay, si.az, s, rx, ry, rz, psi);
+    // the samples have no pressure acceleration: a_i . G
+    __device__ __forceinline__ void wall_add(Acc& a, const Self& si, const Sph&, const float4& g) const {
+        a.ap += dot_g(si.ax, si.ay, si.az, g);
     }
     __device__ __forceinline__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         const float alpha_i = fac_m[t], rho_i = dens_m[t], src_i = src_m[t];
@@ -978,20 +957,37 @@ struct PressureUpdate {
     }
 };
 
-// A policy with the wall samples of FE_BOUNDARY_PARTICLES: after its fluid pairs, every
-// wall sample within reach of the particle adds the policy's wall_pair term.
+// A policy with the wall samples of FE_BOUNDARY_PARTICLES: after its fluid pairs it adds its
+// wall_add term of the particle's (G, rho_w).
 template <class P>
 struct Walled : P {
-    Walls w;
+    const float4* wall_g;
     static constexpr bool kWalls = true;
 };
 
-template <class P>
-__device__ __forceinline__ void wall_terms(const P& pol, typename P::Acc& acc, const typename P::Self& si, const Sph& s,
-                                           const float4& p) {
-    for_each_wall_sample(pol.w, s.h, p.x, p.y, p.z, [&](float rx, float ry, float rz, uint32_t b) {
-        pol.wall_pair(acc, si, s, rx, ry, rz, pol.w.psi[b]);
+// (G, rho_w) of every particle by schedule position: G = sum psi grad W over the samples with
+// r^2 <= h^2 and r > 1e-6, rho_w = sum psi W over those with r^2 <= h^2, each summed in the
+// enumeration order of for_each_wall_sample.
+__global__ __launch_bounds__(kBlock) void k_wall_terms(Walls w, Sph s, const float4* __restrict__ pos_m, uint32_t n,
+                                                       float4* __restrict__ wall_g) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const float4 p = pos_m[t];
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f, rho = 0.0f;
+    for_each_wall_sample(w, s.h, p.x, p.y, p.z, [&](float rx, float ry, float rz, uint32_t b) {
+        const float psi = w.psi[b];
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const bool in = !(r2 > s.hh);
+        const float r = sqrt_exact(r2);
+        const float q = q_of(r, s);
+        rho += in ? psi * w_of_q(q, s) : 0.0f;
+        const bool gok = in && r > 1e-6f;
+        const float g = grad_of_q(r, q, s);
+        gx += gok ? psi * (g * rx) : 0.0f;
+        gy += gok ? psi * (g * ry) : 0.0f;
+        gz += gok ? psi * (g * rz) : 0.0f;
     });
+    wall_g[t] = make_float4(gx, gy, gz, rho);
 }
 
 // Boundary mass of every wall sample: psi_b = rho0 / sum_k W(x_b - x_k) over the samples k
@@ -1102,7 +1098,7 @@ __device__ __forceinline__ unsigned long long gather_one(const P& pol, const Sph
             pol.pair(acc, si, s, pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, pol.load(jt, geo_of(pol, pos_m, jt)));
         });
     }
-    if constexpr (P::kWalls) wall_terms(pol, acc, si, s, pi);
+    if constexpr (P::kWalls) pol.wall_add(acc, si, s, pol.wall_g[t]);
     return pol.finish(i, t, acc, si, s);
 }
 
@@ -1456,6 +1452,25 @@ hipError_t ensure_walls(fe_ctx* c) {
     }
     std::memcpy(c->walls_key, key, sizeof(key));
     c->walls_ready = true;
+    c->wall_g_fresh = false;
+    return hipSuccess;
+}
+
+// (G, rho_w) of the current positions (schedule order), computed once per neighbour search
+// or sample change and read by every gather until then.
+hipError_t ensure_wall_g(fe_ctx* c) {
+    if (!c->walls_ready) return hipErrorInvalidValue;
+    if (c->wall_g_fresh) return hipSuccess;
+    hipError_t e = need(c, kMirrorPos);
+    if (e != hipSuccess) return e;
+    if (!c->wall_g && (e = hipMalloc(&c->wall_g, sizeof(float4) * (size_t)c->n)) != hipSuccess) return e;
+    {
+        Launch L(c, "wall_terms");
+        hipLaunchKernelGGL(k_wall_terms, grid_for(c->n), dim3(kBlock), 0, c->stream, walls_of(c, nullptr), sph_of(c),
+                           c->pos_m, c->n, c->wall_g);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    c->wall_g_fresh = true;
     return hipSuccess;
 }
 
@@ -1465,10 +1480,11 @@ template <class P>
 hipError_t run_walled(fe_ctx* c, const char* name, const P& pol, const unsigned long long* prev = nullptr,
                       unsigned long long thr = 0, uint32_t kind = 0, unsigned long long* slot_out = nullptr) {
     hipError_t e = ensure_walls(c);
+    if (e == hipSuccess) e = ensure_wall_g(c);
     if (e != hipSuccess) return e;
     Walled<P> wp;
     static_cast<P&>(wp) = pol;
-    wp.w = walls_of(c, nullptr);
+    wp.wall_g = c->wall_g;
     Launch L(c, name);
     launch_gather<Walled<P>, false, 256>(c, wp, prev, thr, kind, slot_out);
     return hipGetLastError();

@@ -180,6 +180,10 @@ struct fe_ctx {
     size_t walls_cap = 0;
     float walls_key[9] = {};
     bool walls_ready = false;
+    // per schedule position: (sum psi grad W, sum psi W) over the samples, for the current
+    // positions and samples (cleared by every neighbour search and position upload)
+    float4* wall_g = nullptr;
+    bool wall_g_fresh = false;
 
     // GpuPhysicsData fields
     float4* pos_a = nullptr;

@@ -408,17 +408,29 @@ const WallGeom* walls(State<R>& s) {
     return &geom;
 }
 
-// The samples' terms of a velocity divergence, psi_b (v - 0) . grad W_ib (samples at rest),
-// added one by one to the particle's running sum `acc` after its fluid pairs.
+// A particle's sums over the wall samples, fixed while the positions are: G = sum psi grad W
+// over the samples with r^2 <= h^2 and r > 1e-6, rho_w = sum psi W over those with r^2 <= h^2,
+// in enumeration order. The samples are at rest, so they add v . G to a velocity divergence,
+// a . G to A p, (p / rho^2) G to the pressure acceleration (mirrored pressure), and G and
+// rho_w to the density pass's gradient sum and density.
 template <class R>
-void wall_vdot(const State<R>& s, const WallGeom& w, R h, const V3<R>& p, const V3<R>& v, R& acc) {
+struct WallTerms {
+    V3<R> g{R(0), R(0), R(0)};
+    R rho = R(0);
+};
+
+template <class R>
+WallTerms<R> wall_terms(const State<R>& s, const WallGeom& w, R h, const V3<R>& p) {
     const R hh = h * h;
+    WallTerms<R> t;
     for_each_wall_sample(w, h, p, [&](const V3<R>& rv, uint32_t b) {
         R r2 = dot(rv, rv);
         if (r2 > hh) return;
         R r = std::sqrt(r2);
-        if (r > Lit<R>::eps6) acc += s.wall_psi[b] * dot(v, kernel_grad(rv, r, h));
+        t.rho += s.wall_psi[b] * kernel_w(r, h);
+        if (r > Lit<R>::eps6) t.g = add(t.g, scale(s.wall_psi[b], kernel_grad(rv, r, h)));
     });
+    return t;
 }
 
 template <class R>
@@ -446,14 +458,11 @@ void density_alpha(State<R>& s) {
                 sum_grad_sq += dot(mg, mg);
             }
         });
-        if (wg) for_each_wall_sample(*wg, h, pi, [&](const V3<R>& rv, uint32_t b) {
-            // a sample: psi W to the density, psi grad W to the gradient sum (it does not move)
-            R r2 = dot(rv, rv);
-            if (r2 > hh) return;
-            R r = std::sqrt(r2);
-            density += s.wall_psi[b] * kernel_w(r, h);
-            if (r > Lit<R>::eps6) grad_sum = add(grad_sum, scale(s.wall_psi[b], kernel_grad(rv, r, h)));
-        });
+        if (wg) {
+            const WallTerms<R> wt = wall_terms(s, *wg, h, pi);
+            density += wt.rho;
+            grad_sum = add(grad_sum, wt.g);
+        }
         s.dens[i] = density;
         R alpha = sum_grad_sq + dot(grad_sum, grad_sum);
         s.fac[i] = alpha > Lit<R>::eps6 ? alpha : R(0);
@@ -516,7 +525,7 @@ void density_source(State<R>& s) {
                 div += mass * dot(vd, g);
             }
         });
-        if (wg) wall_vdot(s, *wg, h, pi, vi, div);
+        if (wg) div += dot(vi, wall_terms(s, *wg, h, pi).g);
         R source = R(0);
         if (dt > Lit<R>::eps6) source = ((rho0 - rho_i) / dt) - div;
         s.src[i] = source;
@@ -553,13 +562,7 @@ void pressure_force(State<R>& s) {
                 }
             }
         });
-        if (wg) for_each_wall_sample(*wg, h, pi, [&](const V3<R>& rv, uint32_t b) {
-            // a sample mirrors the particle's pressure: psi (p_i / rho_i^2) grad W
-            R r2 = dot(rv, rv);
-            if (r2 > hh) return;
-            R r = std::sqrt(r2);
-            if (r > Lit<R>::eps6) acc = add(acc, scale(s.wall_psi[b] * p_rho_i, kernel_grad(rv, r, h)));
-        });
+        if (wg) acc = add(acc, scale(p_rho_i, wall_terms(s, *wg, h, pi).g));
         s.put4(s.pacc, i, V3<R>{-acc.x, -acc.y, -acc.z}, R(0));
     }
 }
@@ -597,7 +600,7 @@ uint64_t pressure_update(State<R>& s) {
                 sum_ap += mass * dot(ad, g);
             }
         });
-        if (wg) wall_vdot(s, *wg, h, pi, ai, sum_ap);
+        if (wg) sum_ap += dot(ai, wall_terms(s, *wg, h, pi).g);
         R ap = dt * sum_ap;
         R err = src_i - ap;
         resid += residual_q((float)err, (float)dt, s.p.target_density);
@@ -677,7 +680,7 @@ void divergence_source(State<R>& s) {
                 div += mass * dot(vd, g);
             }
         });
-        if (wg) wall_vdot(s, *wg, h, pi, vi, div);
+        if (wg) div += dot(vi, wall_terms(s, *wg, h, pi).g);
         s.src[i] = -div;
     }
 }
