@@ -4,7 +4,8 @@
 // pressure from 12 features of its neighbourhood. The prediction replaces the initial
 // pressure of the density solve (zero in the reference, density_source_term.comp:73) or of
 // the divergence solve (the density solve's pressures in the reference), and plain Jacobi
-// then runs to the same error thresholds, so the solver stays the judge of the result.
+// then runs to the same error thresholds, so the solver still decides when a result is good
+// enough.
 //
 // Arithmetic: BF16 weights and activations, FP32 accumulation, on the matrix cores
 // (v_mfma_f32_16x16x32_bf16, 16 rows x 16 columns x 32 k per instruction). Operand lane
