@@ -79,6 +79,7 @@ def parse():
     ap.add_argument("--no-splat", action="store_true", help="skip the density-splat (render hand-off) measurement")
     ap.add_argument("--no-consistent", action="store_true", help="skip the consistent divergence-order measurement")
     ap.add_argument("--no-boundary", action="store_true", help="skip the wall boundary-particles measurement")
+    ap.add_argument("--no-app", action="store_true", help="skip the app-scene frame measurement")
     ap.add_argument("--learned-collect", type=int, default=100, help="frames sampled before training")
     ap.add_argument("--learned-eval", type=int, default=400, help="frames compared after training")
     ap.add_argument("--copy-probe", action="store_true",
@@ -257,6 +258,8 @@ def main():
     sim.close()
     if not args.no_consistent:
         out["consistent_divergence"] = divergence_order_measurement(fe, w, opts, args.steps, args.warmup, tuned)
+    if not args.no_app:
+        out["app_frame"] = app_frame_measurement(fe, scene, not args.no_cpu_baseline)
     if not args.no_boundary:
         out["boundary_particles"] = boundary_measurement(fe, w, opts, args.steps, args.warmup)
     if not args.no_learned and args.solver == "converge":
@@ -325,6 +328,54 @@ def divergence_order_measurement(fe, w, opts, steps, warmup, tuned=None):
         out[name] = {"steps_per_s": round(steps / (ms / 1e3), 3),
                      "sweeps_per_step": {"density": round(d / max(frames, 1), 3),
                                          "divergence": round(v / max(frames, 1), 3)}}
+    return out
+
+
+def app_frame_measurement(fe, scene, with_cpu, warmup=20, frames=200):
+    """The reference app's own frame (src/core/scene.rs:14-86, src/renderer/mod.rs:227-391): its
+    25,000-particle scene, the min.x wall oscillating 0.3 m at 0.5 Hz, fixed 4/4 sweeps, radix
+    sort, dt = max_dt = 5 ms, and the raymarching density splat (128^3) after each frame.
+    Frames/s over `frames` frames (HIP events), with and without the splat."""
+    w = scene.app_scene()
+    o = fe.default_options()
+    o.sort_algorithm = fe.FE_SORT_RADIX
+    out = {"workload": f"{w.name}, moving wall {w.wall_wave[0]} m / {w.wall_wave[1]} Hz, fixed 4/4 sweeps",
+           "frames": f"{warmup} warm-up + {frames} timed"}
+    for splat in (False, True):
+        sim = fe.Simulation(w.params, w.positions, device=0, options=o)
+        sim.set_wall(fe.CollisionBox.from_params(w.params, *w.wall_wave))
+        for _ in range(warmup):
+            sim.step()
+            if splat:
+                sim.splat_density()
+        sim.sync()
+        sim.timer_start()
+        for _ in range(frames):
+            sim.step()
+            if splat:
+                sim.splat_density()
+        ms = sim.timer_stop()
+        wall = sim.wall()
+        sim.close()
+        out["with_splat" if splat else "physics_only"] = {"frames_per_s": round(frames / (ms / 1e3), 2),
+                                                           "ms_per_frame": round(ms / frames, 4)}
+        out["wall_min_x_after"] = round(float(wall.min[0]), 5)
+    if with_cpu:
+        # the oracle on the host cores: the same frames with the moving wall, physics only
+        from oracle import oracle
+        cores = min(os.cpu_count() or 1, 16)
+        oracle.set_threads(cores)
+        orc = oracle.Oracle(w.params, w.positions, options=o)
+        orc.set_wall(oracle.make_wall(w.params.box_min[:3], w.params.box_max[:3], *w.wall_wave))
+        n = 0
+        t0 = time.perf_counter()
+        while n < frames and time.perf_counter() - t0 < 5.0:
+            orc.step()
+            n += 1
+        el = time.perf_counter() - t0
+        orc.close()
+        out["cpu_baseline"] = {"value": round(n / el, 2), "unit": "frames/s", "cores": cores, "kind": "port",
+                               "sample": f"{n} physics frames of the same scene from rest by the oracle, {el:.1f} s"}
     return out
 
 
