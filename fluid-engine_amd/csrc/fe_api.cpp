@@ -720,6 +720,8 @@ fe_status fe_set_params(fe_ctx* c, const fe_params* p) {
     FE_HIP(fe::exact_math_flags(c, *p, &fast), "exact arithmetic check");
     // the densities of the last density_alpha used the old mass
     if (std::memcmp(&p->particle_mass, &c->params.particle_mass, sizeof(float)) != 0) c->da_fresh = false;
+    // the list holds the pairs within the old h
+    if (std::memcmp(&p->smoothing_radius, &c->params.smoothing_radius, sizeof(float)) != 0) c->list_valid = false;
     c->params = *p;
     c->exact_fast = fast;
     c->morton_bits = morton_bits_for(*p);
@@ -1287,6 +1289,8 @@ fe_status fe_batch_set_params(fe_ctx* c, uint32_t scene, const fe_params* p) {
         return fail(FE_ERR_INVALID_ARGUMENT, "scenes advance in lockstep: dt and sweep counts must stay equal");
     DeviceGuard g(c->device);
     drop_graph(c);
+    if (std::memcmp(&p->smoothing_radius, &c->scene_params[scene].smoothing_radius, sizeof(float)) != 0)
+        c->list_valid = false;   // the list holds the pairs within the old h
     c->scene_params[scene] = *p;
     c->ns_fresh = c->da_fresh = false;   // h may have changed
     c->mirrors &= ~fe::kMirrorMrho;      // holds the old mass

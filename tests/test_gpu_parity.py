@@ -249,6 +249,33 @@ def test_list_fallbacks_match_oracle(oracle_mod):
         assert_exact("stale-list " + name, sim.download(name), orc.download(name))
 
 
+@pytest.mark.parametrize("scale", [0.9, 1.1])
+def test_new_smoothing_radius_between_stages(scale, oracle_mod):
+    """A new h between stages (fe_set_params after density_alpha): the listed pairs were
+    those within the old h, so the following gathers walk the buckets with the new h, as
+    the oracle does, in place of reusing the list."""
+    w = scene.dam_break_32k(seed=5)
+    sim, orc = pair(w, opts(), oracle_mod)
+    vb = np.zeros((w.n, 4), np.float32)
+    vb[:, :3] = np.random.default_rng(7).normal(0, 0.05, (w.n, 3)).astype(np.float32)
+    for o in (sim, orc):
+        o.neighbor_search()
+        o.density_alpha()
+        o.upload("velocity_b", vb)
+    p = fe.FeParams.from_buffer_copy(sim.params)
+    p.smoothing_radius = F(p.smoothing_radius) * F(scale)
+    for o in (sim, orc):
+        o.set_params(p)
+        o.viscosity()
+        o.density_source()
+        o.pressure_force()
+        o.pressure_update()
+        o.pressure_force()
+    for name in ("velocity_a", "source_terms", "pressures", "pressure_accelerations"):
+        assert_exact(f"h x {scale} {name}", sim.download(name), orc.download(name))
+    assert np.count_nonzero(orc.download("pressure_accelerations")[:, :3]) > 0
+
+
 def test_converge_mode_iterations_match(oracle_mod):
     w = scene.dam_break_32k(seed=2)
     o = opts(mode=fe.FE_SOLVER_CONVERGE, maxit=100)
