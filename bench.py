@@ -80,6 +80,8 @@ def parse():
     ap.add_argument("--no-consistent", action="store_true", help="skip the consistent divergence-order measurement")
     ap.add_argument("--no-boundary", action="store_true", help="skip the wall boundary-particles measurement")
     ap.add_argument("--no-app", action="store_true", help="skip the app-scene frame measurement")
+    ap.add_argument("--no-search-4m", action="store_true",
+                    help="skip the 2^22-particle neighbour search + density_alpha measurement")
     ap.add_argument("--learned-collect", type=int, default=100, help="frames sampled before training")
     ap.add_argument("--learned-eval", type=int, default=400, help="frames compared after training")
     ap.add_argument("--copy-probe", action="store_true",
@@ -268,7 +270,66 @@ def main():
         out["batched_scenes"] = batched_measurement(fe, scene, args.batch_frames)
     if not args.no_sort_bench:
         out["isolated_sort_2p22"] = sort_measurement(fe, scene)
+    if not args.no_search_4m:
+        out["search_4m"] = search_measurement(fe, scene, args.no_cpu_baseline)
     print(json.dumps(out), flush=True)
+
+
+# Algorithmic bytes per particle of one neighbour search (hash 20, four-pass sort 68, fill +
+# offsets 8.5, reorder 52) and of density + alpha (28), T = N.
+SEARCH_BYTES = 148.5
+DENSITY_ALPHA_BYTES = 28.0
+
+
+def search_measurement(fe, scene, no_cpu, warmup=3, reps=20):
+    """The neighbour search and density + alpha of the 2^22-particle block (256 x 128 x 128,
+    spacing 0.02 m, in a 6 x 3 x 3 m box, T = 2^22): hash -> sort -> grid_start -> reorder
+    (plus the gather schedule) and the neighbour list + density_alpha, each stage timed alone
+    with HIP events on the context's stream, median of `reps` after `warmup`. The roofline
+    prices the reference's algorithmic bytes of those stages (the schedule and the list are
+    this build's own extra work inside the same time)."""
+    w = scene.block_4m(seed=0)
+    o = fe.default_options()
+    o.sort_algorithm = fe.FE_SORT_RADIX
+    sim = fe.Simulation(w.params, w.positions, device=0, options=o)
+    ns, da = [], []
+    for r in range(warmup + reps):
+        sim.sync()
+        sim.timer_start()
+        sim.neighbor_search()
+        a = sim.timer_stop()
+        sim.timer_start()
+        sim.density_alpha()
+        b = sim.timer_stop()
+        if r >= warmup:
+            ns.append(a)
+            da.append(b)
+    sim.close()
+    med = lambda v: sorted(v)[len(v) // 2]
+    t_ns, t_da = med(ns), med(da)
+    t = med([a + b for a, b in zip(ns, da)])
+    byts = (SEARCH_BYTES + DENSITY_ALPHA_BYTES) * w.n
+    out = {"workload": f"{w.name} (256 x 128 x 128 block, 6 x 3 x 3 m box)", "particles": w.n, "sort": "radix",
+           "protocol": f"median of {reps} after {warmup} warm-up, HIP events per stage",
+           "neighbor_search_ms": round(t_ns, 4), "density_alpha_ms": round(t_da, 4), "total_ms": round(t, 4),
+           "particles_per_s": round(w.n / (t * 1e-3), 1),
+           "roofline": {"bound": "hbm", "achieved": round(byts / (t * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(byts / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "algo_bytes_per_particle": SEARCH_BYTES + DENSITY_ALPHA_BYTES,
+                        "hbm_floor_ms": round(byts / (HBM_PEAK_GBS * 1e9) * 1e3, 4)}}
+    if not no_cpu:
+        from oracle import oracle
+        cores = min(os.cpu_count() or 1, 16)
+        oracle.set_threads(cores)
+        orc = oracle.Oracle(w.params, w.positions, options=o)
+        t0 = time.perf_counter()
+        orc.neighbor_search()
+        orc.density_alpha()
+        el = time.perf_counter() - t0
+        orc.close()
+        out["cpu_baseline"] = {"value": round(1e3 * el, 1), "unit": "ms per search + density_alpha", "cores": cores,
+                               "kind": "port", "sample": "one search + density_alpha of the same block by the oracle"}
+    return out
 
 
 def sort_measurement(fe, scene, log2n=22, warmup=3, reps=15):

@@ -345,3 +345,24 @@ def test_dam_break_1m_neighbor_search_and_density(oracle_mod):
     orc.density_alpha()
     assert_close("1M densities", sim.download("densities"), orc.download("densities"))
     assert_close("1M factors", sim.download("factors"), orc.download("factors"))
+
+
+def test_block_4m_neighbor_search_and_density(oracle_mod):
+    """The 2^22-particle block (256 x 128 x 128 in a 6 x 3 x 3 m box, T = 2^22): hash, sort,
+    grid_start, reorder, densities and factors bit for bit, in both sorts."""
+    w = scene.block_4m(seed=0)
+    for sort in (fe.FE_SORT_RADIX, fe.FE_SORT_BITONIC):
+        print(f"[gpu parity] {w.name} n={w.n} sort={sort}", flush=True)
+        orc = oracle_mod.Oracle(w.params, w.positions, options=opts(sort=sort))
+        try:
+            orc.neighbor_search()
+            orc.density_alpha()
+            want = {name: orc.download(name) for name in
+                    ("grid_entries", "grid_start", "position_b", "velocity_b", "densities", "factors")}
+        finally:
+            orc.close()
+        with fe.Simulation(w.params, w.positions, device=0, options=opts(sort=sort)) as sim:
+            sim.neighbor_search()
+            sim.density_alpha()
+            for name, ref in want.items():
+                assert_exact(f"4M sort {sort} {name}", sim.download(name), ref)
