@@ -63,7 +63,7 @@ LDS_PEAK_TBS = 128 * 256 * 2.4e9 / 1e12
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="dam_break_1m", choices=["dam_break_1m", "dam_break_32k", "block_4m"])
     ap.add_argument("--solver", default="converge", choices=["converge", "fixed"])
