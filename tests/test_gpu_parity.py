@@ -347,6 +347,30 @@ def test_dam_break_1m_neighbor_search_and_density(oracle_mod):
     assert_close("1M factors", sim.download("factors"), orc.download("factors"))
 
 
+def test_dam_break_1m_frames(oracle_mod):
+    """The benchmark's own frames: the 1M dam break from rest in converge mode with the radix
+    sort, autotuned after the second frame as the benchmark does, bit for bit with the oracle
+    (stats and sweep counts every frame, every field after frames 1 and 3)."""
+    w = scene.dam_break_1m(seed=0)
+    print(f"[gpu parity] {w.name} n={w.n} frames", flush=True)
+    o = opts(mode=fe.FE_SOLVER_CONVERGE, density_tol=1e-3, divergence_tol=1e-2)
+    with fe.Simulation(w.params, w.positions, device=0, options=o) as sim:
+        orc = oracle_mod.Oracle(w.params, w.positions, options=o)
+        try:
+            for frame in range(4):
+                if frame == 2:
+                    sim.autotune_now(reps=3)
+                sim.step()
+                orc.step()
+                assert_exact(f"1M frame {frame} stats", sim.read_stats(), orc.read_stats())
+                assert sim.last_iterations() == orc.last_iterations(), frame
+                if frame in (1, 3):
+                    for name in FIELDS_STEP + ("grid_entries", "grid_start"):
+                        assert_exact(f"1M frame {frame} {name}", sim.download(name), orc.download(name))
+        finally:
+            orc.close()
+
+
 def test_block_4m_neighbor_search_and_density(oracle_mod):
     """The 2^22-particle block (256 x 128 x 128 in a 6 x 3 x 3 m box, T = 2^22): hash, sort,
     grid_start, reorder, densities and factors bit for bit, in both sorts."""
