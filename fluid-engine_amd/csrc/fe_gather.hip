@@ -542,21 +542,28 @@ struct DensityAlpha {
     __device__ Self self(uint32_t, uint32_t, const float4&) const { return {}; }
     __device__ Cand load(uint32_t, const float4&) const { return {}; }
     __device__ Acc zero() const { return {0.f, 0.f, 0.f, 0.f, 0.f}; }
-    __device__ __forceinline__ void pair(Acc& a, const Self&, const Sph& s, float rx, float ry, float rz,
-                                         const Cand&) const {
+    // The pair's terms; returns its gradient scalar g (grad W = g r) and whether the pair
+    // has a gradient term (r^2 <= h^2 and r > 1e-6).
+    __device__ __forceinline__ static float terms(Acc& a, const Sph& s, float rx, float ry, float rz, bool& gok) {
         const float r2 = rx * rx + ry * ry + rz * rz;
         const bool in = !(r2 > s.hh);
         const float r = sqrt_exact(r2);
         const float q = q_of(r, s);
         const float w = w_of_q(q, s);
         a.rho += in ? s.mass * w : 0.0f;
-        const bool gok = in && r > 1e-6f;
+        gok = in && r > 1e-6f;
         const float g = grad_of_q(r, q, s);
         const float mx = s.mass * (g * rx), my = s.mass * (g * ry), mz = s.mass * (g * rz);
         a.gx += gok ? mx : 0.0f;
         a.gy += gok ? my : 0.0f;
         a.gz += gok ? mz : 0.0f;
         a.sgs += gok ? mx * mx + my * my + mz * mz : 0.0f;
+        return g;
+    }
+    __device__ __forceinline__ void pair(Acc& a, const Self&, const Sph& s, float rx, float ry, float rz,
+                                         const Cand&) const {
+        bool gok;
+        (void)terms(a, s, rx, ry, rz, gok);
     }
     // the samples' density and gradient sum (they do not move: no term in the sum of squares)
     __device__ __forceinline__ void wall_add(Acc& a, const Self&, const Sph&, const float4& g) const {
@@ -696,6 +703,42 @@ struct DivergenceSource {
         const float p = pres[i];
         pres_m[t] = p;
         pos_w[t].w = prho_of(p, dens[i]);
+        return 0;
+    }
+};
+
+// density_and_alpha.comp:14-72 and divergence_source_term.comp:14-62 in one pass over the
+// pairs of the post-integrate search, which the reference runs back to back: the divergence
+// source reads no neighbour density, and its pair term uses the same gradient scalar. Where
+// DensityAlpha has a gradient term (gok) its g equals DivergenceSource's exact_grad of the
+// same r^2 (sqrt_exact and sqrt_grad agree from 2^-96 on, q_of and q_big from r = 1e-6 on),
+// and elsewhere both add nothing, so every sum is the two kernels' own, in list order. The
+// particle's own density comes from the accumulator instead of the densities field.
+struct DensityAlphaDivergence {
+    DensityAlpha da;
+    DivergenceSource dv;
+    struct Self { float4 v; };
+    struct Cand { float4 v; };
+    struct Acc { DensityAlpha::Acc a; float div; };
+    static constexpr bool kPaGeo = false;
+    static constexpr bool kWalls = false;
+    __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {dv.va_m[t]}; }
+    __device__ Cand load(uint32_t jt, const float4&) const { return {dv.va_m[jt]}; }
+    __device__ Acc zero() const { return {da.zero(), 0.f}; }
+    __device__ __forceinline__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
+                                         const Cand& c) const {
+        bool gok;
+        const float g = DensityAlpha::terms(a.a, s, rx, ry, rz, gok);
+        const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
+        a.div += gok ? s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz)) : 0.0f;
+    }
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
+        (void)da.finish(i, t, a.a, DensityAlpha::Self{}, s);
+        dv.src[i] = -a.div;
+        dv.src_m[t] = -a.div;
+        const float p = dv.pres[i];
+        dv.pres_m[t] = p;
+        dv.pos_w[t].w = prho_of(p, a.a.rho);
         return 0;
     }
 };
@@ -1501,6 +1544,22 @@ hipError_t launch_density_alpha_only(fe_ctx* c) {
 hipError_t launch_density_alpha(fe_ctx* c) {
     hipError_t e = launch_build_list(c);
     return e == hipSuccess ? launch_density_alpha_only(c) : e;
+}
+
+bool density_alpha_divergence_fusable(const fe_ctx* c) {
+    return !walls_on(c) && c->options.divergence_order == FE_DIVERGENCE_REFERENCE && !learn_features_wanted(c, 1);
+}
+
+hipError_t launch_density_alpha_divergence_source(fe_ctx* c) {
+    hipError_t e = launch_build_list(c);   // also brings the position mirror
+    if (e == hipSuccess) e = need(c, kMirrorPos | kMirrorVelA);
+    if (e != hipSuccess) return e;
+    DensityAlphaDivergence pol{{c->dens, c->fac, c->dens_m, c->fac_m, c->vb_m},
+                               {c->va_m, c->dens, c->pres, c->src, c->pos_m, c->src_m, c->pres_m}};
+    e = run_list(c, "density_alpha_divergence_source", kTuneDensityAlpha, pol);
+    c->mirrors |= kMirrorDens | kMirrorFac | kMirrorMrho | kMirrorPrho | kMirrorSrc | kMirrorPres;
+    c->da_fresh = c->ns_fresh;
+    return e;
 }
 
 hipError_t launch_viscosity(fe_ctx* c) {
