@@ -1547,15 +1547,19 @@ hipError_t launch_density_alpha(fe_ctx* c) {
 }
 
 bool density_alpha_divergence_fusable(const fe_ctx* c) {
-    return !walls_on(c) && c->options.divergence_order == FE_DIVERGENCE_REFERENCE && !learn_features_wanted(c, 1);
+    return !walls_on(c) && !learn_features_wanted(c, 1);
 }
 
 hipError_t launch_density_alpha_divergence_source(fe_ctx* c) {
+    // The consistent order's velocities are the velocity_b mirror, whose w this pass writes
+    // (m / rho); the pass reads only its xyz.
+    const bool consistent = c->options.divergence_order == FE_DIVERGENCE_CONSISTENT;
     hipError_t e = launch_build_list(c);   // also brings the position mirror
-    if (e == hipSuccess) e = need(c, kMirrorPos | kMirrorVelA);
+    if (e == hipSuccess) e = need(c, kMirrorPos | (consistent ? kMirrorVelB : kMirrorVelA));
     if (e != hipSuccess) return e;
     DensityAlphaDivergence pol{{c->dens, c->fac, c->dens_m, c->fac_m, c->vb_m},
-                               {c->va_m, c->dens, c->pres, c->src, c->pos_m, c->src_m, c->pres_m}};
+                               {consistent ? c->vb_m : c->va_m, c->dens, c->pres, c->src, c->pos_m, c->src_m,
+                                c->pres_m}};
     e = run_list(c, "density_alpha_divergence_source", kTuneDensityAlpha, pol);
     c->mirrors |= kMirrorDens | kMirrorFac | kMirrorMrho | kMirrorPrho | kMirrorSrc | kMirrorPres;
     c->da_fresh = c->ns_fresh;
