@@ -732,6 +732,11 @@ struct DensityAlphaDivergence {
         const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
         a.div += gok ? s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz)) : 0.0f;
     }
+    // each kernel's wall term, after its fluid pairs
+    __device__ __forceinline__ void wall_add(Acc& a, const Self& si, const Sph& s, const float4& g) const {
+        da.wall_add(a.a, DensityAlpha::Self{}, s, g);
+        a.div += dot_g(si.v.x, si.v.y, si.v.z, g);
+    }
     __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
         (void)da.finish(i, t, a.a, DensityAlpha::Self{}, s);
         dv.src[i] = -a.div;
@@ -1546,9 +1551,7 @@ hipError_t launch_density_alpha(fe_ctx* c) {
     return e == hipSuccess ? launch_density_alpha_only(c) : e;
 }
 
-bool density_alpha_divergence_fusable(const fe_ctx* c) {
-    return !walls_on(c) && !learn_features_wanted(c, 1);
-}
+bool density_alpha_divergence_fusable(const fe_ctx* c) { return !learn_features_wanted(c, 1); }
 
 hipError_t launch_density_alpha_divergence_source(fe_ctx* c) {
     // The consistent order's velocities are the velocity_b mirror, whose w this pass writes
@@ -1560,7 +1563,8 @@ hipError_t launch_density_alpha_divergence_source(fe_ctx* c) {
     DensityAlphaDivergence pol{{c->dens, c->fac, c->dens_m, c->fac_m, c->vb_m},
                                {consistent ? c->vb_m : c->va_m, c->dens, c->pres, c->src, c->pos_m, c->src_m,
                                 c->pres_m}};
-    e = run_list(c, "density_alpha_divergence_source", kTuneDensityAlpha, pol);
+    e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source", pol)
+                    : run_list(c, "density_alpha_divergence_source", kTuneDensityAlpha, pol);
     c->mirrors |= kMirrorDens | kMirrorFac | kMirrorMrho | kMirrorPrho | kMirrorSrc | kMirrorPres;
     c->da_fresh = c->ns_fresh;
     return e;
