@@ -183,8 +183,9 @@ constexpr int PK_B1 = kMlpPackedOps * 64 * 4, PK_B2 = PK_B1 + H, PK_B3 = PK_B2 +
 static_assert(PK_END <= kMlpPackedFloats, "packed layout");
 
 // GELU of the inference kernel: piecewise linear through GELU at z = -8 + k / 128,
-// k = 0 .. 2048 (entry k holds the value at node k and the rise to node k + 1), with the end
-// segments extended linearly (GELU(z) is 0 below -8 and z above 8 to within 1e-14).
+// k = 0 .. 2048, with the end segments extended linearly (GELU(z) is 0 below -8 and z above 8
+// to within 1e-14). In the table coordinate u = (z + 8) 128, entry k holds segment k as
+// (intercept, slope): GELU ~ intercept + slope u on [k, k + 1].
 // Interpolation error <= (1/128)^2 / 8 * max|GELU''| < 7e-6, far below a BF16 step of the
 // activations it feeds; the table is built from gelu() above.
 constexpr int kGeluNodes = 2048;
@@ -194,7 +195,8 @@ __global__ __launch_bounds__(256) void k_gelu_table(float2* __restrict__ table) 
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < kGeluNodes; k += gridDim.x * blockDim.x) {
         const double z0 = (double)kGeluLo + (double)k / kGeluScale, z1 = z0 + 1.0 / kGeluScale;
         const float g0 = gelu((float)z0), g1 = gelu((float)z1);
-        table[k] = make_float2(g0, g1 - g0);
+        const float rise = g1 - g0;
+        table[k] = make_float2((float)((double)g0 - (double)rise * k), rise);
     }
 }
 
@@ -202,10 +204,12 @@ static_assert(PK_END + 2 * kGeluNodes <= kMlpPackedFloats, "packed layout with t
 
 __device__ __forceinline__ float gelu_lut(const float2* __restrict__ lut, float z) {
     const float u = fmaf(z, kGeluScale, -kGeluLo * kGeluScale);
-    int k = (int)floorf(u);   // v_cvt_i32_f32: NaN -> 0, saturating; clamped below in any case
+    // the segment of u; truncation equals floor once negative u clamp to segment 0
+    // (v_cvt_i32_f32: NaN -> 0, saturating)
+    int k = (int)u;
     k = k < 0 ? 0 : (k > kGeluNodes - 1 ? kGeluNodes - 1 : k);
     const float2 e = lut[k];
-    return fmaf(e.y, u - (float)k, e.x);
+    return fmaf(e.y, u, e.x);
 }
 
 // Operand g: layer 1 tiles g = 0..3 (column tile ct = g), layer 2 g = 4 + 2 ct + kc, layer 3
