@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ para
     }
 }
 
-constexpr int kInferWaves = 4;
+constexpr int kInferWaves = 8;
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 u) { return __builtin_bit_cast(bf16x8, u); }
 
@@ -283,7 +283,7 @@ __device__ __forceinline__ void store_act(const float2* lut, const float* sb, co
 // y goes to yraw[s] and the inputs are used as given; otherwise p = max(0, y target_std +
 // target_mean) goes to pressures[sched[s]], to its mirror pres_m[s], and p / rho^2 to the
 // w of the position mirror pos_m[s].
-__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ packed,
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const float* __restrict__ packed,
                                                    const float2* __restrict__ gelu_table,
                                                    const float* __restrict__ feat, uint32_t n, uint32_t raw,
                                                    const uint32_t* __restrict__ sched, const float* __restrict__ dens,
@@ -292,19 +292,18 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ pac
     __shared__ alignas(16) float sf[PK_END - PK_B1];
     __shared__ alignas(16) __bf16 act[kInferWaves][16 * H];
     __shared__ float2 lut[kGeluNodes];
+    // the layer 2 and 3 operands (groups 4 .. 19), one 16-byte row per lane: in LDS rather
+    // than in 64 VGPRs per lane, which would allow only two waves per SIMD
+    __shared__ uint4 sops[16 * 64];
     for (int e = threadIdx.x; e < PK_END - PK_B1; e += blockDim.x) sf[e] = packed[PK_B1 + e];
     for (int e = threadIdx.x; e < kGeluNodes; e += blockDim.x) lut[e] = gelu_table[e];
+    const uint4* ops = reinterpret_cast<const uint4*>(packed);
+    for (int e = threadIdx.x; e < 16 * 64; e += blockDim.x) sops[e] = ops[4 * 64 + e];
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int c = l & 15, q = l >> 4;
-    const uint4* ops = reinterpret_cast<const uint4*>(packed);
-    bf16x8 A1[4], A2[8], A3[8];
+    bf16x8 A1[4];
 #pragma unroll
     for (int g = 0; g < 4; g++) A1[g] = as_bf16x8(ops[g * 64 + l]);
-#pragma unroll
-    for (int g = 0; g < 8; g++) {
-        A2[g] = as_bf16x8(ops[(4 + g) * 64 + l]);
-        A3[g] = as_bf16x8(ops[(12 + g) * 64 + l]);
-    }
     __syncthreads();
     float mu[8], is[8];
 #pragma unroll
@@ -340,8 +339,8 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ pac
             wave_lds_fence();   // both operands read before the row is overwritten
 #pragma unroll
             for (int ct = 0; ct < 4; ct++) {
-                const bf16x8& a0 = layer == 0 ? A2[2 * ct] : A3[2 * ct];
-                const bf16x8& a1 = layer == 0 ? A2[2 * ct + 1] : A3[2 * ct + 1];
+                const bf16x8 a0 = as_bf16x8(sops[(8 * layer + 2 * ct) * 64 + l]);
+                const bf16x8 a1 = as_bf16x8(sops[(8 * layer + 2 * ct + 1) * 64 + l]);
                 acc[ct] = mfma(a1, B1, mfma(a0, B0, f32x4{0.f, 0.f, 0.f, 0.f}));
             }
             if (layer == 0) {
