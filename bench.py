@@ -4,8 +4,9 @@
 One step = one physics frame of Renderer::step with max_dt == dt = 1 ms (frame-start
 neighbour search + density/alpha, viscosity, density Jacobi solve to 0.1 % compression
 error, integration, neighbour rebuild, density/alpha, divergence Jacobi solve to 1 %,
-divergence integration, stats; at most 100 sweeps per solve). Inputs are resident in HBM
-before the timed region; the region is bracketed by HIP events on the context's stream.
+divergence integration, stats; at most 100 sweeps per solve). By default 500 steps are timed
+after 5 warm-up steps from rest. Inputs are resident in HBM before the timed region; the region
+is bracketed by HIP events on the context's stream.
 
 Prints one JSON line: steps/s (value), particle-updates/s, Jacobi sweeps per step, the
 roofline of the dominant kernel (algorithmic bytes / event-timed average launch, and its
