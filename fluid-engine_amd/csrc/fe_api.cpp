@@ -477,12 +477,7 @@ fe_status step(fe_ctx* c, float max_dt) {
         FE_HIP(fe::launch_pressure_integration(c), "pressure_integration");
         t += c->params.dt;
         if ((s = neighbor_search(c)) != FE_OK) return s;
-        if (fe::density_alpha_divergence_fusable(c)) {
-            FE_HIP(fe::launch_density_alpha_divergence_source(c), "density_alpha + divergence_source");
-        } else {
-            FE_HIP(fe::launch_density_alpha(c), "density_alpha");
-            FE_HIP(fe::launch_divergence_source(c), "divergence_source");
-        }
+        FE_HIP(fe::launch_density_alpha_divergence_source(c), "density_alpha + divergence_source");
         if ((s = solve(c, 1, vit, c->options.divergence_tol, nullptr)) != FE_OK) return s;
         FE_HIP(fe::launch_divergence_integration(c), "divergence_integration");
         if (!(c->params.dt > 0.0f)) break;   // a non-positive dt would never end the frame

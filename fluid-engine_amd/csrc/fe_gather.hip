@@ -542,12 +542,13 @@ struct DensityAlpha {
     __device__ Self self(uint32_t, uint32_t, const float4&) const { return {}; }
     __device__ Cand load(uint32_t, const float4&) const { return {}; }
     __device__ Acc zero() const { return {0.f, 0.f, 0.f, 0.f, 0.f}; }
-    // The pair's terms; returns its gradient scalar g (grad W = g r) and whether the pair
-    // has a gradient term (r^2 <= h^2 and r > 1e-6).
-    __device__ __forceinline__ static float terms(Acc& a, const Sph& s, float rx, float ry, float rz, bool& gok) {
+    // The pair's terms; returns its gradient scalar g (grad W = g r), whether the pair has a
+    // gradient term (r^2 <= h^2 and r > 1e-6) and its distance r.
+    __device__ __forceinline__ static float terms(Acc& a, const Sph& s, float rx, float ry, float rz, bool& gok,
+                                                  float& r) {
         const float r2 = rx * rx + ry * ry + rz * rz;
         const bool in = !(r2 > s.hh);
-        const float r = sqrt_exact(r2);
+        r = sqrt_exact(r2);
         const float q = q_of(r, s);
         const float w = w_of_q(q, s);
         a.rho += in ? s.mass * w : 0.0f;
@@ -563,7 +564,8 @@ struct DensityAlpha {
     __device__ __forceinline__ void pair(Acc& a, const Self&, const Sph& s, float rx, float ry, float rz,
                                          const Cand&) const {
         bool gok;
-        (void)terms(a, s, rx, ry, rz, gok);
+        float r;
+        (void)terms(a, s, rx, ry, rz, gok, r);
     }
     // the samples' density and gradient sum (they do not move: no term in the sum of squares)
     __device__ __forceinline__ void wall_add(Acc& a, const Self&, const Sph&, const float4& g) const {
@@ -728,7 +730,8 @@ struct DensityAlphaDivergence {
     __device__ __forceinline__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz,
                                          const Cand& c) const {
         bool gok;
-        const float g = DensityAlpha::terms(a.a, s, rx, ry, rz, gok);
+        float r;
+        const float g = DensityAlpha::terms(a.a, s, rx, ry, rz, gok, r);
         const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
         a.div += gok ? s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz)) : 0.0f;
     }
@@ -883,6 +886,51 @@ struct DivergenceSourceFeat {
         return 0;
     }
     uint32_t n_;
+};
+
+// DensityAlphaDivergence plus the warm-start features of the divergence solve, the fused form
+// of DensityAlpha followed by DivergenceSourceFeat: the features take the particle's own
+// density and factor, which this pass writes before it forms them.
+struct DensityAlphaDivergenceFeat {
+    DensityAlpha da;
+    DivergenceSource dv;
+    const float* pprev_m;
+    float* feat;
+    uint32_t n_;
+    struct Self { float4 v; float pp; };
+    struct Cand { float4 v; float pp; };
+    struct Acc { DensityAlpha::Acc a; float div; FeatAcc f; };
+    static constexpr bool kPaGeo = false;
+    static constexpr bool kWalls = false;
+    __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {dv.va_m[t], pprev_m[t]}; }
+    __device__ Cand load(uint32_t jt, const float4&) const { return {dv.va_m[jt], pprev_m[jt]}; }
+    __device__ Acc zero() const { return {da.zero(), 0.f, feat_zero()}; }
+    // the samples enter density, factor and source term; the features describe fluid neighbours
+    __device__ __forceinline__ void wall_add(Acc& a, const Self& si, const Sph& s, const float4& g) const {
+        da.wall_add(a.a, DensityAlpha::Self{}, s, g);
+        a.div += dot_g(si.v.x, si.v.y, si.v.z, g);
+    }
+    __device__ void pair(Acc& a, const Self& si, const Sph& s, float rx, float ry, float rz, const Cand& c) const {
+        bool gok;
+        float r;
+        const float g = DensityAlpha::terms(a.a, s, rx, ry, rz, gok, r);
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 > s.hh) return;
+        const float dx = si.v.x - c.v.x, dy = si.v.y - c.v.y, dz = si.v.z - c.v.z;
+        if (gok) a.div += s.mass * (dx * (g * rx) + dy * (g * ry) + dz * (g * rz));
+        feat_pair(a.f, s, rx, ry, rz, r2, r, dx, dy, dz, c.pp);
+    }
+    __device__ unsigned long long finish(uint32_t i, uint32_t t, const Acc& a, const Self& si, const Sph& s) const {
+        (void)da.finish(i, t, a.a, DensityAlpha::Self{}, s);
+        const float source = -a.div;
+        dv.src[i] = source;
+        dv.src_m[t] = source;
+        const float rho = a.a.rho, p = dv.pres[i];
+        dv.pres_m[t] = p;
+        dv.pos_w[t].w = prho_of(p, rho);
+        feat_write(feat, n_, t, s, a.f, rho, da.fac[i], source, si.pp, s.rho0, s.dt);
+        return 0;
+    }
 };
 
 // pressure_force.comp:15-78. Neighbours supply p_j / rho_j^2 in the w of their position
@@ -1551,8 +1599,6 @@ hipError_t launch_density_alpha(fe_ctx* c) {
     return e == hipSuccess ? launch_density_alpha_only(c) : e;
 }
 
-bool density_alpha_divergence_fusable(const fe_ctx* c) { return !learn_features_wanted(c, 1); }
-
 hipError_t launch_density_alpha_divergence_source(fe_ctx* c) {
     // The consistent order's velocities are the velocity_b mirror, whose w this pass writes
     // (m / rho); the pass reads only its xyz.
@@ -1563,8 +1609,16 @@ hipError_t launch_density_alpha_divergence_source(fe_ctx* c) {
     DensityAlphaDivergence pol{{c->dens, c->fac, c->dens_m, c->fac_m, c->vb_m},
                                {consistent ? c->vb_m : c->va_m, c->dens, c->pres, c->src, c->pos_m, c->src_m,
                                 c->pres_m}};
-    e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source", pol)
-                    : run_list(c, "density_alpha_divergence_source", kTuneDensityAlpha, pol);
+    if (learn_features_wanted(c, 1)) {
+        if ((e = learn_prev(c, 1)) != hipSuccess) return e;
+        DensityAlphaDivergenceFeat fp{pol.da, pol.dv, c->ws.pprev, c->ws.feat, c->n};
+        e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source_feat", fp)
+                        : run_list(c, "density_alpha_divergence_source_feat", kTuneDensityAlpha, fp);
+        learn_features_done(c, 1);
+    } else {
+        e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source", pol)
+                        : run_list(c, "density_alpha_divergence_source", kTuneDensityAlpha, pol);
+    }
     c->mirrors |= kMirrorDens | kMirrorFac | kMirrorMrho | kMirrorPrho | kMirrorSrc | kMirrorPres;
     c->da_fresh = c->ns_fresh;
     return e;

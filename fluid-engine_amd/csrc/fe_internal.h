@@ -332,9 +332,8 @@ hipError_t launch_rebuild_ends(fe_ctx* c);
 hipError_t launch_density_alpha(fe_ctx* c);   // build_list, then the density_alpha gather
 hipError_t launch_build_list(fe_ctx* c);
 hipError_t launch_density_alpha_only(fe_ctx* c);
-// build_list, then density_alpha and divergence_source in one gather (the post-integrate pair
-// of a frame), where the divergence source gathers no warm-start features
-bool density_alpha_divergence_fusable(const fe_ctx* c);
+// build_list, then density_alpha and divergence_source (with the warm-start features when
+// wanted) in one gather: the post-integrate pair of a frame
 hipError_t launch_density_alpha_divergence_source(fe_ctx* c);
 hipError_t launch_viscosity(fe_ctx* c);
 hipError_t launch_density_source(fe_ctx* c);
