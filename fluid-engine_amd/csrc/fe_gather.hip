@@ -1599,28 +1599,55 @@ hipError_t launch_density_alpha(fe_ctx* c) {
     return e == hipSuccess ? launch_density_alpha_only(c) : e;
 }
 
-hipError_t launch_density_alpha_divergence_source(fe_ctx* c) {
+namespace {
+
+DensityAlphaDivergence density_alpha_divergence_of(const fe_ctx* c) {
     // The consistent order's velocities are the velocity_b mirror, whose w this pass writes
     // (m / rho); the pass reads only its xyz.
     const bool consistent = c->options.divergence_order == FE_DIVERGENCE_CONSISTENT;
+    return DensityAlphaDivergence{{c->dens, c->fac, c->dens_m, c->fac_m, c->vb_m},
+                                  {consistent ? c->vb_m : c->va_m, c->dens, c->pres, c->src, c->pos_m, c->src_m,
+                                   c->pres_m}};
+}
+
+hipError_t density_alpha_divergence_mirrors(fe_ctx* c) {
+    const bool consistent = c->options.divergence_order == FE_DIVERGENCE_CONSISTENT;
+    return need(c, kMirrorPos | (consistent ? kMirrorVelB : kMirrorVelA));
+}
+
+void density_alpha_divergence_done(fe_ctx* c) {
+    c->mirrors |= kMirrorDens | kMirrorFac | kMirrorMrho | kMirrorPrho | kMirrorSrc | kMirrorPres;
+    c->da_fresh = c->ns_fresh;
+}
+
+}  // namespace
+
+hipError_t launch_density_alpha_divergence_source(fe_ctx* c) {
     hipError_t e = launch_build_list(c);   // also brings the position mirror
-    if (e == hipSuccess) e = need(c, kMirrorPos | (consistent ? kMirrorVelB : kMirrorVelA));
+    if (e == hipSuccess) e = density_alpha_divergence_mirrors(c);
     if (e != hipSuccess) return e;
-    DensityAlphaDivergence pol{{c->dens, c->fac, c->dens_m, c->fac_m, c->vb_m},
-                               {consistent ? c->vb_m : c->va_m, c->dens, c->pres, c->src, c->pos_m, c->src_m,
-                                c->pres_m}};
+    const DensityAlphaDivergence pol = density_alpha_divergence_of(c);
     if (learn_features_wanted(c, 1)) {
         if ((e = learn_prev(c, 1)) != hipSuccess) return e;
         DensityAlphaDivergenceFeat fp{pol.da, pol.dv, c->ws.pprev, c->ws.feat, c->n};
         e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source_feat", fp)
-                        : run_list(c, "density_alpha_divergence_source_feat", kTuneDensityAlpha, fp);
+                        : run_list(c, "density_alpha_divergence_source_feat", kTuneDensityAlphaDivergence, fp);
         learn_features_done(c, 1);
     } else {
         e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source", pol)
-                        : run_list(c, "density_alpha_divergence_source", kTuneDensityAlpha, pol);
+                        : run_list(c, "density_alpha_divergence_source", kTuneDensityAlphaDivergence, pol);
     }
-    c->mirrors |= kMirrorDens | kMirrorFac | kMirrorMrho | kMirrorPrho | kMirrorSrc | kMirrorPres;
-    c->da_fresh = c->ns_fresh;
+    density_alpha_divergence_done(c);
+    return e;
+}
+
+hipError_t launch_density_alpha_divergence_only(fe_ctx* c) {
+    hipError_t e = density_alpha_divergence_mirrors(c);
+    if (e != hipSuccess) return e;
+    const DensityAlphaDivergence pol = density_alpha_divergence_of(c);
+    e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source", pol)
+                    : run_list(c, "density_alpha_divergence_source", kTuneDensityAlphaDivergence, pol);
+    density_alpha_divergence_done(c);
     return e;
 }
 

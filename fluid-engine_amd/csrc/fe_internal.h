@@ -126,6 +126,7 @@ enum TuneId {
     kTuneBuildList,
     kTunePressureIntegration,
     kTuneDivergenceIntegration,
+    kTuneDensityAlphaDivergence,   // density_alpha + divergence source in one pass
     kTuneCount
 };
 struct TuneCfg {
@@ -335,6 +336,8 @@ hipError_t launch_density_alpha_only(fe_ctx* c);
 // build_list, then density_alpha and divergence_source (with the warm-start features when
 // wanted) in one gather: the post-integrate pair of a frame
 hipError_t launch_density_alpha_divergence_source(fe_ctx* c);
+// the same pass over the current list, without the build and the features (the autotuner's)
+hipError_t launch_density_alpha_divergence_only(fe_ctx* c);
 hipError_t launch_viscosity(fe_ctx* c);
 hipError_t launch_density_source(fe_ctx* c);
 // Jacobi sweep kernels. prev: the previous sweep's residual slot (nullptr: no early exit);

@@ -714,7 +714,8 @@ hipError_t launch_pack_start(fe_ctx* c, uint32_t* out) {
 const char* tune_name(int id) {
     static const char* names[kTuneCount] = {"density_alpha",     "viscosity",      "density_source",
                                             "divergence_source", "pressure_force", "pressure_update",
-                                            "build_list",        "pressure_integration", "divergence_integration"};
+                                            "build_list",        "pressure_integration", "divergence_integration",
+                                            "density_alpha_divergence_source"};
     return id >= 0 && id < kTuneCount ? names[id] : "";
 }
 

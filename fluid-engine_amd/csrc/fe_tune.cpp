@@ -29,6 +29,7 @@ hipError_t launch_tuned(fe_ctx* c, int id) {
         case fe::kTuneBuildList: return fe::launch_build_list(c);
         case fe::kTunePressureIntegration: return fe::launch_pressure_integration(c);
         case fe::kTuneDivergenceIntegration: return fe::launch_divergence_integration(c);
+        case fe::kTuneDensityAlphaDivergence: return fe::launch_density_alpha_divergence_only(c);
         default: return hipErrorInvalidValue;
     }
 }

@@ -33,7 +33,8 @@ def test_autotune_keeps_results(tmp_path):
         tun.run(2)
         res = tun.autotune(str(tmp_path / "tune.json"), reps=5)
         assert set(res) == {"density_alpha", "viscosity", "density_source", "divergence_source", "pressure_force",
-                            "pressure_update", "build_list", "pressure_integration", "divergence_integration"}
+                            "pressure_update", "build_list", "pressure_integration", "divergence_integration",
+                            "density_alpha_divergence_source"}
         for k, v in res.items():
             assert v["median_us"] > 0 and v["default_us"] > 0, k
             assert v["median_us"] <= v["default_us"] * 1.0001, k
