@@ -14,7 +14,9 @@ from ctypes import POINTER, c_float, c_int, c_int32, c_size_t, c_uint32, c_uint6
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "libfe_oracle.so")
+# FE_ORACLE_LIB names another build of fe_oracle.cpp (tests/test_oracle_mutants.py loads
+# variants with a planted mistake to show that the pins catch it).
+LIB = os.environ.get("FE_ORACLE_LIB") or os.path.join(HERE, "libfe_oracle.so")
 
 _lib = None
 
