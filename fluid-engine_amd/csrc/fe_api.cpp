@@ -425,8 +425,7 @@ fe_status neighbor_search(fe_ctx* c) {
     c->ends_dirty = false;
     c->list_valid = false;
     c->ws.feat_fresh = false;
-    // gather schedule: particles in Morton order of their cells
-    FE_HIP(fe::launch_morton_keys(c), "morton keys");
+    // gather schedule: particles in Morton order of their cells (keys from offsets/reorder)
     FE_HIP(fe::radix_sort_pairs(c, c->mkeys, c->sched, c->n, c->morton_bits + c->scene_bits, "schedule"),
            "schedule sort");
     FE_HIP(fe::launch_schedule_mirrors(c), "schedule mirrors");
@@ -614,7 +613,7 @@ static fe_status alloc_ctx(fe_ctx* c, const float* xyz) {
     ALLOC(c->win_delta, sizeof(uint32_t) * fe::kWinMaxRuns * nwin);
     ALLOC(c->nbr16, sizeof(uint16_t) * fe::kMaxNbr * fe::kWin * nwin);   // whole windows
 #undef ALLOC
-    e = fe::sort_scratch_alloc(&c->sort, c->T);
+    e = fe::sort_scratch_alloc(&c->sort, c->T, c->stream);
     if (e != hipSuccess) return hip_fail(e, "sort scratch");
     e = hipHostMalloc((void**)&c->host_stats, 16, hipHostMallocDefault);
     if (e == hipSuccess)

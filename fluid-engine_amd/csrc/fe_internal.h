@@ -43,10 +43,18 @@ struct ProfileTotal {
     double total_ms = 0.0;
 };
 
-// Scratch of the radix sort: per-tile digit counts and the digit totals of one pass.
+// Scratch of the one-sweep radix sort (fe_sort.hip): the digit counts of every pass over all
+// keys (zero between sorts: a sort's last tile clears them), the look-back word of every
+// (pass, tile of 2048 keys, digit), and the tile tickets of every pass followed by an error
+// word that a look-back sets if it ever stops waiting.
+constexpr uint32_t kSortMaxPasses = 4;
+constexpr uint32_t kSortMaxKeys = (1u << 30) - 1u;   // counts share a word with a 2-bit state
+constexpr uint32_t kSortTileKeys = 4096;             // keys per tile
+constexpr uint32_t kSortGroupTiles = 32;             // tiles per look-back group
 struct SortScratch {
-    uint32_t* counts = nullptr;   // 256 * max_tiles
-    uint32_t* totals = nullptr;   // 256
+    uint32_t* hist = nullptr;     // kSortMaxPasses x 256
+    uint32_t* status = nullptr;   // sort_status_words(max_tiles): tile words, then group words
+    uint32_t* ctrl = nullptr;     // kSortMaxPasses tickets, then the error word
     uint32_t max_tiles = 0;
 };
 
@@ -305,7 +313,16 @@ hipError_t radix_sort_pairs(fe_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uin
                             const char* tag);
 // The bitonic network of bitonic_sort.comp over all T entries, in place in keys[0]/vals[0].
 hipError_t bitonic_sort(fe_ctx* c);
-hipError_t sort_scratch_alloc(SortScratch* s, uint32_t max_n);
+// Stable LSD radix sort of n pairs on bits [0, key_bits) on `stream` with explicit scratch;
+// result in keys[0]/vals[0] (buffers swapped as needed). prof (may be null) records launches.
+hipError_t radix_sort_core(const SortScratch& s, hipStream_t stream, uint32_t* keys[2], uint32_t* vals[2],
+                           uint32_t n, uint32_t key_bits, fe_ctx* prof);
+// Tiles of a sort of max_n keys, and the look-back words a scratch for that many tiles holds.
+uint32_t sort_max_tiles(uint32_t max_n);
+size_t sort_status_words(uint32_t max_tiles);
+hipError_t sort_scratch_alloc(SortScratch* s, uint32_t max_n, hipStream_t stream);
+// Zero the digit counts, tickets and error word (one small kernel, no memset).
+hipError_t sort_scratch_clear(const SortScratch& s, hipStream_t stream);
 void sort_scratch_free(SortScratch* s);
 
 // ---- launchers (fe_gather.hip) ----
@@ -323,9 +340,9 @@ hipError_t exact_math_flags(fe_ctx* c, const fe_params& p, uint32_t* flags);
 hipError_t check_exact_arith(hipStream_t st, float h, unsigned long long samples, unsigned long long* dev_bad,
                              unsigned long long counts[6]);
 hipError_t launch_hash(fe_ctx* c);
+// Bucket ranges, position_b / velocity_b, and the Morton keys of position_b's cells with the
+// identity schedule payload (mkeys[0], sched[0]).
 hipError_t launch_offsets_reorder(fe_ctx* c);
-// Morton keys of the cells of position_b and the identity schedule payload.
-hipError_t launch_morton_keys(fe_ctx* c);
 hipError_t launch_iota(fe_ctx* c, uint32_t* p, uint32_t count);
 hipError_t launch_fill_ranges(fe_ctx* c);
 hipError_t launch_offsets_only(fe_ctx* c);

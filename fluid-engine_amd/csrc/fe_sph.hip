@@ -49,29 +49,10 @@ __global__ __launch_bounds__(kBlock) void k_hash(const float4* __restrict__ pos,
     }
 }
 
-// grid_offsets.comp:20-32 (bucket starts) fused with reorder.comp:37-45 (pos/vel gather)
-// and the bucket ends the gathers use. Entries at i >= n are sentinels.
-__global__ __launch_bounds__(kBlock) void k_offsets_reorder(const uint32_t* __restrict__ keys,
-                                                            const uint32_t* __restrict__ vals, uint32_t n,
-                                                            uint2* __restrict__ range,
-                                                            const float4* __restrict__ pos_a,
-                                                            const float4* __restrict__ vel_a,
-                                                            float4* __restrict__ pos_b, float4* __restrict__ vel_b) {
-    uint32_t i = gid();
-    if (i >= n) return;
-    uint32_t key = keys[i];
-    if (key != kSentinel) {
-        if (i == 0 || keys[i - 1] != key) range[key].x = i;
-        if (i + 1 == n || keys[i + 1] != key) range[key].y = i + 1;
-    }
-    uint32_t src = vals[i];
-    pos_b[i] = pos_a[src];
-    vel_b[i] = vel_a[src];
-}
-
-// Morton key of each particle's cell (the low `bits` bits of each coordinate relative to
-// the box's minimum cell, interleaved), paired with the particle index: sorted, it orders
-// the gathers' work so that neighbouring cells are processed close together in time.
+// Morton key of a particle's cell (the low `bits` bits of each coordinate relative to the
+// box's minimum cell, interleaved): sorted with the particle's hash-order index, it orders
+// the gathers' work so that neighbouring cells are processed close together in time. In a
+// batch the scene number sits above the 3 x bits Morton bits, which keeps scenes apart.
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {
     v &= 0x3FFu;
     v = (v | (v << 16)) & 0x030000FFu;
@@ -81,20 +62,41 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
     return v;
 }
 
-// In a batch the scene number sits above the 3 x bits Morton bits, which keeps scenes apart.
-__global__ __launch_bounds__(kBlock) void k_morton_keys(const float4* __restrict__ pos, uint32_t n, Sph s0, Scenes sc,
-                                                        uint32_t bits, uint32_t* __restrict__ keys,
-                                                        uint32_t* __restrict__ idx) {
-    const uint32_t i = gid();
-    if (i >= n) return;
-    const Sph s = scene_at(sc, s0, i);
+__device__ __forceinline__ uint32_t morton_key(const Scenes& sc, const Sph& s, uint32_t i, const float4& p,
+                                               uint32_t bits) {
     const uint32_t mask = (1u << bits) - 1u;
-    const float4 p = pos[i];
     const int3 c = cell_of(p.x, p.y, p.z, s.inv_h);
     const uint32_t x = (uint32_t)(c.x - s.c0x) & mask, y = (uint32_t)(c.y - s.c0y) & mask,
                    z = (uint32_t)(c.z - s.c0z) & mask;
-    keys[i] = (scene_id(sc, i) << (3u * bits)) | spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
-    idx[i] = i;
+    return (scene_id(sc, i) << (3u * bits)) | spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+}
+
+// grid_offsets.comp:20-32 (bucket starts) fused with reorder.comp:37-45 (pos/vel gather),
+// the bucket ends the gathers use, and the Morton key of each reordered particle's cell with
+// its hash-order index (the input of the gather schedule's sort). Entries at i >= n are
+// sentinels.
+__global__ __launch_bounds__(kBlock) void k_offsets_reorder(const uint32_t* __restrict__ keys,
+                                                            const uint32_t* __restrict__ vals, uint32_t n,
+                                                            uint2* __restrict__ range,
+                                                            const float4* __restrict__ pos_a,
+                                                            const float4* __restrict__ vel_a,
+                                                            float4* __restrict__ pos_b, float4* __restrict__ vel_b,
+                                                            Sph s0, Scenes sc, uint32_t bits,
+                                                            uint32_t* __restrict__ mkeys,
+                                                            uint32_t* __restrict__ midx) {
+    uint32_t i = gid();
+    if (i >= n) return;
+    uint32_t key = keys[i];
+    if (key != kSentinel) {
+        if (i == 0 || keys[i - 1] != key) range[key].x = i;
+        if (i + 1 == n || keys[i + 1] != key) range[key].y = i + 1;
+    }
+    uint32_t src = vals[i];
+    const float4 p = pos_a[src];
+    pos_b[i] = p;
+    vel_b[i] = vel_a[src];
+    mkeys[i] = morton_key(sc, scene_at(sc, s0, i), i, p, bits);
+    midx[i] = i;
 }
 
 __global__ __launch_bounds__(kBlock) void k_iota(uint32_t* __restrict__ p, uint32_t count) {
@@ -443,7 +445,8 @@ hipError_t launch_hash(fe_ctx* c) {
 hipError_t launch_offsets_reorder(fe_ctx* c) {
     Launch L(c, "offsets_reorder");
     hipLaunchKernelGGL(k_offsets_reorder, grid_for(c->n), dim3(kBlock), 0, c->stream, c->keys[0], c->vals[0], c->n,
-                       c->range, c->pos_a, c->vel_a, c->pos_b, c->vel_b);
+                       c->range, c->pos_a, c->vel_a, c->pos_b, c->vel_b, sph_of(c), scenes_of(c), c->morton_bits / 3,
+                       c->mkeys[0], c->sched[0]);
     return hipGetLastError();
 }
 
@@ -602,13 +605,6 @@ hipError_t exact_math_flags(fe_ctx* c, const fe_params& p, uint32_t* flags) {
     }
     *flags = (ok_h ? kFastDivH : 0u) | (div_box_ok(h) ? kFastDivBox : 0u);
     return hipSuccess;
-}
-
-hipError_t launch_morton_keys(fe_ctx* c) {
-    Launch L(c, "morton_keys");
-    hipLaunchKernelGGL(k_morton_keys, grid_for(c->n), dim3(kBlock), 0, c->stream, c->pos_b, c->n, sph_of(c),
-                       scenes_of(c), c->morton_bits / 3, c->mkeys[0], c->sched[0]);
-    return hipGetLastError();
 }
 
 hipError_t launch_iota(fe_ctx* c, uint32_t* p, uint32_t count) {
