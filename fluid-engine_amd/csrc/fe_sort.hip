@@ -21,8 +21,11 @@
 // Bitonic: the compare-exchange network of bitonic_sort.comp:19-43 as driven by
 // GpuSorter::execute (sorter.rs:73-115); the stages whose pairs stay inside a 2048-entry
 // block run together in LDS. The network is the same, so the (unstable) result is the same.
+#include <array>
+#include <mutex>
 #include <optional>
 
+#include "fe_abi.h"
 #include "fe_internal.h"
 
 namespace fe {
@@ -538,4 +541,92 @@ hipError_t bitonic_sort(fe_ctx* c) {
     return hipGetLastError();
 }
 
+// ---- the sort without allocation (fe_sort_pairs_noalloc) --------------------------------
+// The same radix sort on caller-owned entries, with every buffer beyond them in one fixed
+// device array per GPU (the split keys and values with their ping-pong partners, the digit
+// counts, tickets and look-back words of a sort of FE_SORT_NOALLOC_MAX_PAIRS pairs).
+namespace {
+constexpr uint32_t kNaPairs = FE_SORT_NOALLOC_MAX_PAIRS;
+constexpr uint32_t kNaTiles = kNaPairs / kMinTile;
+constexpr size_t kNaHist = 4 * (size_t)kNaPairs;                 // word offsets in the array
+constexpr size_t kNaCtrl = kNaHist + kSortMaxPasses * 256u;
+constexpr size_t kNaStatus = kNaCtrl + 64;
+constexpr size_t kNaWords = kNaStatus + (size_t)kSortMaxPasses * (kNaTiles + (kNaTiles + kGroup - 1) / kGroup) * 256u;
+static_assert(kNaPairs % kMinTile == 0, "whole tiles");
+static_assert(kNaWords * 4 == FE_SORT_NOALLOC_SCRATCH_BYTES, "documented scratch size");
+}  // namespace
+
+static __device__ __align__(256) uint32_t scratch[kNaWords];
+
+namespace {
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mutex;
+std::array<uint32_t*, kNaMaxDevices> g_na_ptr{};   // device address of `scratch` per device
+
+__global__ __launch_bounds__(256) void k_split_pairs(const uint2* __restrict__ e, uint32_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const uint2 v = e[i];
+        keys[i] = v.x;
+        vals[i] = v.y;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_join_pairs(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                    uint2* __restrict__ e, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) e[i] = make_uint2(keys[i], vals[i]);
+}
+}  // namespace
+
+}  // namespace fe
+
+extern "C" fe_status fe_sort_pairs_noalloc(int device, uint32_t* entries, uint32_t n, void* stream) {
+    using namespace fe;
+    if (n > kNaPairs)
+        return fail(FE_ERR_INVALID_ARGUMENT, "fe_sort_pairs_noalloc: n = " + std::to_string(n) +
+                                                 " exceeds FE_SORT_NOALLOC_MAX_PAIRS (" + std::to_string(kNaPairs) + ")");
+    if (!entries && n > 0) return fail(FE_ERR_INVALID_ARGUMENT, "fe_sort_pairs_noalloc: null entries");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(FE_ERR_DEVICE, "no HIP device");
+    if (device < 0 && hipGetDevice(&device) != hipSuccess) return fail(FE_ERR_DEVICE, "hipGetDevice failed");
+    if (device >= count || device >= kNaMaxDevices)
+        return fail(FE_ERR_INVALID_ARGUMENT, "fe_sort_pairs_noalloc: bad device " + std::to_string(device));
+    DeviceGuard g(device);
+    uint32_t* base;
+    {
+        std::lock_guard<std::mutex> lock(g_na_mutex);
+        if (!g_na_ptr[device]) {
+            // once per device, with that device current: its copy of the array
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+            g_na_ptr[device] = static_cast<uint32_t*>(p);
+        }
+        base = g_na_ptr[device];
+    }
+    if (n <= 1) return FE_OK;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    SortScratch s;
+    s.hist = base + kNaHist;
+    s.ctrl = base + kNaCtrl;
+    s.status = base + kNaStatus;
+    s.max_tiles = kNaTiles;
+    uint32_t* keys[2] = {base, base + 2 * (size_t)kNaPairs};
+    uint32_t* vals[2] = {base + kNaPairs, base + 3 * (size_t)kNaPairs};
+    // the array keeps the previous call's contents: the counts and tickets start from zero
+    FE_HIP(sort_scratch_clear(s, st), "fe_sort_pairs_noalloc clear");
+    const uint32_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_split_pairs, dim3(blocks), dim3(256), 0, st, reinterpret_cast<const uint2*>(entries), keys[0],
+                       vals[0], n);
+    FE_HIP(hipGetLastError(), "fe_sort_pairs_noalloc split");
+    FE_HIP(radix_sort_core(s, st, keys, vals, n, 32, nullptr), "fe_sort_pairs_noalloc sort");
+    hipLaunchKernelGGL(k_join_pairs, dim3(blocks), dim3(256), 0, st, keys[0], vals[0], reinterpret_cast<uint2*>(entries),
+                       n);
+    FE_HIP(hipGetLastError(), "fe_sort_pairs_noalloc join");
+    return FE_OK;
+}
+
+namespace fe {
 }  // namespace fe

@@ -141,6 +141,7 @@ _SIGNATURES = [
     ("fe_device_ptr", [c_void_p, c_int, POINTER(c_void_p)]),
     ("fe_neighbor_search", [c_void_p]),
     ("fe_sort", [c_void_p]),
+    ("fe_sort_pairs_noalloc", [c_int, c_void_p, c_uint32, c_void_p]),
     ("fe_density_alpha", [c_void_p]),
     ("fe_viscosity", [c_void_p]),
     ("fe_density_source", [c_void_p]),

@@ -178,6 +178,20 @@ fe_status fe_neighbor_search(fe_ctx* ctx);
 /* The sort stage alone on GRID_ENTRIES (RadixSorter::execute sorter.rs:325-455 or
  * GpuSorter::execute sorter.rs:73-115, by options.sort_algorithm). */
 fe_status fe_sort(fe_ctx* ctx);
+/* fe_sort's radix sort on caller-owned device memory, allocating nothing. No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). `entries` is a device
+ * pointer to n interleaved {uint32 hash, uint32 index} pairs (the GRID_ENTRIES layout); they
+ * are sorted in place, stably by all 32 bits of the hash, on HIP stream `stream` (a
+ * hipStream_t; NULL is the null stream) of HIP device `device` (the current one if negative),
+ * asynchronously. The result equals fe_sort's on the same pairs. n <= FE_SORT_NOALLOC_MAX_PAIRS,
+ * else FE_ERR_INVALID_ARGUMENT and nothing runs. Every buffer beyond the entries lives in one
+ * fixed device array of FE_SORT_NOALLOC_SCRATCH_BYTES per GPU, which that GPU holds until the
+ * process ends; every call on a GPU shares it, so calls that overlap on one GPU (other streams,
+ * other threads) race. */
+#define FE_SORT_NOALLOC_MAX_PAIRS 4194304u
+#define FE_SORT_NOALLOC_SCRATCH_BYTES 71438592u
+fe_status fe_sort_pairs_noalloc(int device, uint32_t* entries, uint32_t n, void* stream);
 fe_status fe_density_alpha(fe_ctx* ctx);            /* density_and_alpha.comp */
 fe_status fe_viscosity(fe_ctx* ctx);                /* viscosity.comp */
 fe_status fe_density_source(fe_ctx* ctx);           /* density_source_term.comp */
