@@ -567,49 +567,77 @@ def splat_measurement(fe, sim, w, no_cpu, warmup=3, reps=20):
     return out
 
 
-def learned_comparison(fe, w, opts, collect, evaluate, tuned=None):
-    """Learned warm start on the same workload: two contexts from the same initial state; the
-    second samples 2048 (features, final pressure) pairs per solve over the first `collect`
-    frames, both networks are trained on the GPU (2000 Adam steps, batch 4096), and then both
-    contexts run `evaluate` more frames, one from the reference start, one from the learned
-    start, to the same 0.1 % / 1 % thresholds. Reports sweeps per frame and frames per second
-    of each over the evaluated frames."""
-    ref = fe.Simulation(w.params, w.positions, device=0, options=opts)
-    lrn = fe.Simulation(w.params, w.positions, device=0, options=opts)
-    for k, v in (tuned or {}).items():   # both contexts run the tuned configurations
-        ref.set_tune(k, v["block"], v["items"])
-        lrn.set_tune(k, v["block"], v["items"])
-    lrn.learn_collect(2048, 1 << 18)
-    ref.run(collect)
-    lrn.run(collect)
-    lrn.sync()
-    lrn.learn_collect(0)
+def learned_comparison(fe, w, opts, collect, evaluate, tuned=None, report_frames=100):
+    """Learned warm start on the same workload. A context samples 2048 (features, final
+    pressure) pairs per solve over the first `collect` frames and both networks are trained on
+    the GPU (2000 Adam steps, batch 4096). Then, for each exit residual (FE_RESIDUAL_TWO_SIDED,
+    the fair comparison: an over-predicted pressure is not taken as converged; and the
+    compression-only X1/X2 residual), a context from the reference start and one from the
+    learned start run `evaluate` frames from the same state at frame `collect` to the same
+    0.1 % / 1 % thresholds (sweeps per frame and frames per second, HIP events), then
+    `report_frames` more frames with the exit-residual report on (fe_solve_report, outside the
+    timed frames)."""
+    def make(collecting):
+        sim = fe.Simulation(w.params, w.positions, device=0, options=opts)
+        for k, v in (tuned or {}).items():   # every context runs the tuned configurations
+            sim.set_tune(k, v["block"], v["items"])
+        if collecting:
+            sim.learn_collect(2048, 1 << 18)   # also keeps each particle's previous pressures
+        sim.run(collect)
+        sim.sync()
+        if collecting:
+            sim.learn_collect(0)
+        return sim
+
+    trainer = make(True)
     t0 = time.perf_counter()
-    l0 = lrn.learn_train(fe.FE_NET_DENSITY, steps=2000, batch=4096, lr=2e-3, seed=1)
-    l1 = lrn.learn_train(fe.FE_NET_DIVERGENCE, steps=2000, batch=4096, lr=2e-3, seed=2)
+    l0 = trainer.learn_train(fe.FE_NET_DENSITY, steps=2000, batch=4096, lr=2e-3, seed=1)
+    l1 = trainer.learn_train(fe.FE_NET_DIVERGENCE, steps=2000, batch=4096, lr=2e-3, seed=2)
     train_s = time.perf_counter() - t0
-    lo = fe.FeOptions.from_buffer_copy(opts)
-    lo.warm_start = fe.FE_WARM_START_LEARNED
-    lrn.set_options(lo)
-    res = {}
-    for name, sim in (("reference", ref), ("learned", lrn)):
-        sim.reset_iteration_totals()
-        sim.timer_start()
-        sim.run(evaluate)
-        ms = sim.timer_stop()
-        d, v, f = sim.iteration_totals()
-        res[name] = {"steps_per_s": round(evaluate / (ms / 1e3), 3),
-                     "sweeps_per_step": {"density": round(d / f, 3), "divergence": round(v / f, 3)},
-                     "mean_abs_density_error": round(float(sim.read_stats()[1]) / w.n, 3)}
-    # inference kernel, event-timed over a few learned frames
-    lrn.profile(True)
-    lrn.profile_reset()
-    lrn.run(5)
-    prof = lrn.profile_results()
-    lrn.profile(False)
-    ref.close()
-    lrn.close()
-    rs, ls = res["reference"]["sweeps_per_step"], res["learned"]["sweeps_per_step"]
+    nets = {net: trainer.learn_weights(net) for net in (fe.FE_NET_DENSITY, fe.FE_NET_DIVERGENCE)}
+    prof = {}
+    modes = {}
+    for mode, resid in (("two_sided", fe.FE_RESIDUAL_TWO_SIDED), ("compression", fe.FE_RESIDUAL_COMPRESSION)):
+        res = {}
+        for name in ("reference", "learned"):
+            if name == "learned" and mode == "two_sided":
+                sim, trainer = trainer, None          # continues from its own sampled frames
+            else:
+                sim = make(name == "learned")
+            o = fe.FeOptions.from_buffer_copy(opts)
+            o.residual = resid
+            if name == "learned":
+                for net, (prm, norm) in nets.items():
+                    sim.learn_set_weights(net, prm, norm)
+                o.warm_start = fe.FE_WARM_START_LEARNED
+            sim.set_options(o)
+            sim.reset_iteration_totals()
+            sim.timer_start()
+            sim.run(evaluate)
+            ms = sim.timer_stop()
+            d, v, f = sim.iteration_totals()
+            r = {"steps_per_s": round(evaluate / (ms / 1e3), 3),
+                 "sweeps_per_step": {"density": round(d / f, 3), "divergence": round(v / f, 3)},
+                 "mean_abs_density_error": round(float(sim.read_stats()[1]) / w.n, 3)}
+            sim.solve_report_enable(True)
+            sim.run(report_frames)
+            r["exit_residual"] = {kind: {k: (round(x, 7) if isinstance(x, float) else x)
+                                         for k, x in sim.solve_report(kid).items()}
+                                  for kind, kid in (("density", 0), ("divergence", 1))}
+            sim.solve_report_enable(False)
+            if name == "learned" and mode == "two_sided":
+                # inference kernel, event-timed over a few learned frames
+                sim.profile(True)
+                sim.profile_reset()
+                sim.run(5)
+                prof = sim.profile_results()
+                sim.profile(False)
+            sim.close()
+            res[name] = r
+        rs, ls = res["reference"]["sweeps_per_step"], res["learned"]["sweeps_per_step"]
+        res["sweep_reduction"] = round((rs["density"] + rs["divergence"]) / (ls["density"] + ls["divergence"]), 3)
+        res["speedup"] = round(res["learned"]["steps_per_s"] / res["reference"]["steps_per_s"], 3)
+        modes[mode] = res
     out = {
         "workload": f"{w.name}: frames {collect}-{collect + evaluate - 1} after sampling frames 0-{collect - 1}",
         "samples_per_network": collect * 2048,
@@ -617,10 +645,14 @@ def learned_comparison(fe, w, opts, collect, evaluate, tuned=None):
                      "density_loss_first_last": [round(float(l0[:20].mean()), 4), round(float(l0[-50:].mean()), 4)],
                      "divergence_loss_first_last": [round(float(l1[:20].mean()), 4),
                                                     round(float(l1[-50:].mean()), 4)]},
-        "reference": res["reference"],
-        "learned": res["learned"],
-        "sweep_reduction": round((rs["density"] + rs["divergence"]) / (ls["density"] + ls["divergence"]), 3),
-        "speedup": round(res["learned"]["steps_per_s"] / res["reference"]["steps_per_s"], 3),
+        "exit_residual_frames": f"{collect + evaluate}-{collect + evaluate + report_frames - 1}, "
+                                "units of the relative rate r dt / rho0",
+        # the fair comparison: both starts stop on the two-sided residual
+        "reference": modes["two_sided"]["reference"],
+        "learned": modes["two_sided"]["learned"],
+        "sweep_reduction": modes["two_sided"]["sweep_reduction"],
+        "speedup": modes["two_sided"]["speedup"],
+        "compression_residual": modes["compression"],
     }
     if "ws_infer" in prof:
         launches, total_ms = prof["ws_infer"]

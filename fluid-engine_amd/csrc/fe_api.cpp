@@ -64,7 +64,8 @@ void free_ctx(fe_ctx* c) {
                     (void*)c->sched[0], (void*)c->sched[1], (void*)c->mkeys[0], (void*)c->mkeys[1],
                     (void*)c->inv, (void*)c->pos_m, (void*)c->vb_m, (void*)c->va_m, (void*)c->dens_m,
                     (void*)c->pa_m, (void*)c->pa2_m, (void*)c->fac_m, (void*)c->src_m, (void*)c->pres_m,
-                    (void*)c->win_meta, (void*)c->win_round, (void*)c->win_delta, (void*)c->nbr16})
+                    (void*)c->win_meta, (void*)c->win_round, (void*)c->win_delta, (void*)c->nbr16,
+                    (void*)c->report_err, (void*)c->report})
         if (p) (void)hipFree(p);
     fe::sort_scratch_free(&c->sort);
     fe::learn_free(c);
@@ -291,6 +292,11 @@ fe_status solve(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t* 
     if (s == FE_OK) {
         c->defer_sweep_writes = true;
         s = jacobi(c, kind, fixed_iters, tol, iterations);
+        if (s == FE_OK && c->report_on) {
+            hipError_t e = fe::launch_residual_report(c, kind);
+            if (e != hipSuccess) s = hip_fail(e, "residual report");
+            c->report_solves[kind]++;
+        }
         c->defer_sweep_writes = false;
         hipError_t e = fe::flush_sweep_writes(c);
         if (s == FE_OK && e != hipSuccess) s = hip_fail(e, "sweep write-back");
@@ -768,6 +774,7 @@ fe_status fe_set_options(fe_ctx* c, const fe_options* o) {
                     "warm_start LEARNED needs a network: fe_learn_train or fe_learn_set_weights first");
     if (o->solver_mode == FE_SOLVER_CONVERGE && (o->max_iterations < 1 || o->max_iterations > fe::kMaxSolveIters))
         return fail(FE_ERR_INVALID_ARGUMENT, "max_iterations must be in [1, 1024]");
+    if (o->residual > FE_RESIDUAL_TWO_SIDED) return fail(FE_ERR_INVALID_ARGUMENT, "unknown residual");
     if (c->nscenes > 1 && (o->sort_algorithm != FE_SORT_RADIX || o->use_cfl || o->use_solver_error_threshold ||
                            o->warm_start != FE_WARM_START_REFERENCE))
         return fail(FE_ERR_UNSUPPORTED,
@@ -993,6 +1000,40 @@ fe_status fe_iteration_totals(fe_ctx* c, uint64_t* density, uint64_t* divergence
     if (density) *density = c->total_iters[0];
     if (divergence) *divergence = c->total_iters[1];
     if (frames) *frames = c->frames;
+    return FE_OK;
+}
+
+fe_status fe_solve_report_enable(fe_ctx* c, int enable) {
+    FE_CHECK_CTX(c);
+    if (c->nscenes > 1) return fail(FE_ERR_UNSUPPORTED, "solve reports take one scene");
+    DeviceGuard g(c->device);
+    if (enable) {
+        if (!c->report_err) FE_HIP(hipMalloc((void**)&c->report_err, sizeof(float) * (size_t)c->n), "report buffer");
+        if (!c->report) FE_HIP(hipMalloc((void**)&c->report, sizeof(unsigned long long) * 8), "report sums");
+        FE_HIP(hipMemsetAsync(c->report, 0, sizeof(unsigned long long) * 8, c->stream), "report reset");
+        c->report_solves[0] = c->report_solves[1] = 0;
+    }
+    c->report_on = enable != 0;
+    return FE_OK;
+}
+
+fe_status fe_solve_report(fe_ctx* c, int kind, uint32_t* solves, double out[4]) {
+    FE_CHECK_CTX(c);
+    if (kind != 0 && kind != 1) return fail(FE_ERR_INVALID_ARGUMENT, "kind must be 0 (density) or 1 (divergence)");
+    if (!out) return fail(FE_ERR_INVALID_ARGUMENT, "null out");
+    const uint32_t k = c->report ? c->report_solves[kind] : 0;
+    if (solves) *solves = k;
+    for (int j = 0; j < 4; j++) out[j] = 0.0;
+    if (!k) return FE_OK;
+    DeviceGuard g(c->device);
+    unsigned long long h[4];
+    FE_HIP(hipMemcpyAsync(h, c->report + 4 * kind, sizeof(h), hipMemcpyDeviceToHost, c->stream), "report read");
+    FE_HIP(hipStreamSynchronize(c->stream), "report wait");
+    const double unit = 1.0 / 16777216.0;   // the sums hold r dt / rho0 in units of 2^-24
+    out[0] = (double)h[0] * unit / ((double)k * c->n);
+    out[1] = (double)h[3] * unit;
+    out[2] = (double)h[1] * unit / ((double)k * c->n);
+    out[3] = (double)h[2] * unit / ((double)k * c->n);
     return FE_OK;
 }
 

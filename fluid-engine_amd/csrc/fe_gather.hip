@@ -997,6 +997,7 @@ struct PressureUpdate {
     float* pres;
     float* pres_m;
     float4* pos_w;
+    uint32_t two_sided;   // FE_RESIDUAL_TWO_SIDED: |s - Ap| where the entering pressure is > 0
     static constexpr int kParts = 2;
     static constexpr bool kPaGeo = true;
     static constexpr bool kWalls = false;
@@ -1037,7 +1038,7 @@ struct PressureUpdate {
         float p = pres_m[t];
         const float ap = s.dt * a.ap;
         const float err = src_i - ap;
-        const unsigned long long q = residual_q(err, s.dt, s.rho0);
+        const unsigned long long q = residual_q(two_sided && p > 0.0f ? -fabsf(err) : err, s.dt, s.rho0);
         if (alpha_i > 1e-6f && rho_i > 1e-6f) {
             const float a_ii = -(s.dt / (rho_i * rho_i)) * alpha_i;
             if (fabsf(a_ii) > 1e-20f) {
@@ -1052,6 +1053,49 @@ struct PressureUpdate {
         return q;
     }
 };
+
+// The exit report of a solve (no reference counterpart): pressure_update's sum over the
+// listed pairs with the accelerations of the solve's last sweep, keeping r = s - Ap of every
+// particle (by schedule position) instead of updating its pressure.
+struct ResidualReport : PressureUpdate {
+    float* err_out;
+    __device__ __forceinline__ unsigned long long finish(uint32_t, uint32_t t, const Acc& a, const Self&, const Sph& s) const {
+        err_out[t] = src_m[t] - s.dt * a.ap;
+        return 0;
+    }
+};
+
+// Sums of a report: |r|, max(0, r) and max(0, r) where the final pressure is > 0, each as
+// floor(x dt / rho0 2^24), plus the largest |r| term (slots 0..3).
+__global__ __launch_bounds__(kBlock) void k_residual_reduce(const float* __restrict__ err, const float* __restrict__ pres_m,
+                                                            uint32_t n, float dt, float rho0,
+                                                            unsigned long long* __restrict__ out) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    unsigned long long a = 0, e = 0, ea = 0;
+    if (t < n) {
+        const float r = err[t];
+        const float scale = dt / rho0 * 16777216.0f;
+        const float ab = fminf(fabsf(r) * scale, 1.0e15f), ex = fminf(fmaxf(r, 0.0f) * scale, 1.0e15f);
+        a = (unsigned long long)ab;
+        e = (unsigned long long)ex;
+        ea = pres_m[t] > 0.0f ? e : 0ull;
+    }
+    unsigned long long mx = a;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        e += __shfl_xor(e, off, 64);
+        ea += __shfl_xor(ea, off, 64);
+        const unsigned long long o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (a) atomicAdd(out, a);
+        if (e) atomicAdd(out + 1, e);
+        if (ea) atomicAdd(out + 2, ea);
+        if (mx) atomicMax(out + 3, mx);
+    }
+}
 
 // A policy with the wall samples of FE_BOUNDARY_PARTICLES: after its fluid pairs it adds its
 // wall_add term of the particle's (G, rho_w).
@@ -1714,11 +1758,26 @@ hipError_t launch_pressure_update(fe_ctx* c, unsigned long long* out, const unsi
     hipError_t e = need(c, kMirrorPacc | kMirrorDens | kMirrorFac | kMirrorSrc | kMirrorPres);
     if (e != hipSuccess) return e;
     PressureUpdate pol{c->pa_m,   c->pa2_m,  c->fac_m, c->src_m, c->dens_m, c->defer_sweep_writes ? nullptr : c->pres,
-                       c->pres_m, c->pos_m};
+                       c->pres_m, c->pos_m, c->options.residual == FE_RESIDUAL_TWO_SIDED ? 1u : 0u};
     e = walls_on(c) ? run_walled(c, "pressure_update", pol, prev, thr, kind, out)
                     : run_sweep(c, "pressure_update", kTunePressureUpdate, pol, prev, thr, kind, out);
     c->pending_pres |= c->defer_sweep_writes;
     return e;
+}
+
+hipError_t launch_residual_report(fe_ctx* c, int kind) {
+    hipError_t e = need(c, kMirrorPacc | kMirrorDens | kMirrorFac | kMirrorSrc | kMirrorPres);
+    if (e != hipSuccess) return e;
+    ResidualReport pol;
+    static_cast<PressureUpdate&>(pol) = PressureUpdate{c->pa_m, c->pa2_m, c->fac_m, c->src_m, c->dens_m, nullptr,
+                                                       c->pres_m, c->pos_m, 0u};
+    pol.err_out = c->report_err;
+    e = walls_on(c) ? run_walled(c, "residual_report", pol) : run_list(c, "residual_report", kTunePressureUpdate, pol);
+    if (e != hipSuccess) return e;
+    Launch L(c, "residual_reduce");
+    hipLaunchKernelGGL(k_residual_reduce, grid_for(c->n), dim3(kBlock), 0, c->stream, c->report_err, c->pres_m, c->n,
+                       c->params.dt, c->params.target_density, c->report + 4 * kind);
+    return hipGetLastError();
 }
 
 hipError_t flush_sweep_writes(fe_ctx* c) {

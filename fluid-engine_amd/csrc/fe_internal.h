@@ -278,6 +278,13 @@ struct fe_ctx {
     bool have_prev_stats = false;
     uint32_t prev_stats[3] = {0, 0, 0};
 
+    // exit reports of the Jacobi solves (fe_solve_report_enable): r = s - Ap by schedule
+    // position, and per solve kind the sums of k_residual_reduce
+    bool report_on = false;
+    float* report_err = nullptr;
+    unsigned long long* report = nullptr;   // 2 x 4
+    uint32_t report_solves[2] = {0, 0};
+
     uint32_t last_iters[2] = {0, 0};
     bool last_iters_pending[2] = {false, false};
     uint64_t total_iters[2] = {0, 0};
@@ -366,6 +373,9 @@ hipError_t launch_pressure_update(fe_ctx* c, unsigned long long* out, const unsi
 // The hash-ordered pressures / pressure accelerations from their mirrors after sweeps that
 // deferred them (no-op when nothing is pending).
 hipError_t flush_sweep_writes(fe_ctx* c);
+// The exit report of the solve that just ended (kind 0 density, 1 divergence): r = s - Ap of
+// its last sweep, reduced into report[4 kind .. 4 kind + 3].
+hipError_t launch_residual_report(fe_ctx* c, int kind);
 hipError_t launch_pressure_integration(fe_ctx* c);
 hipError_t launch_divergence_source(fe_ctx* c);
 hipError_t launch_divergence_integration(fe_ctx* c);

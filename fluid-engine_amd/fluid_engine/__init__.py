@@ -17,7 +17,7 @@ from . import scene, tuning
 from ._ffi import (EXPORTED_SYMBOLS, FE_MLP_HIDDEN, FE_MLP_INPUTS, FE_MLP_NORM, FE_MLP_PARAMS, FE_NET_DENSITY,
                    FE_NET_DIVERGENCE, FE_SOLVER_CONVERGE, FE_SOLVER_FIXED, FE_SORT_BITONIC, FE_SORT_RADIX,
                    FE_WARM_START_LEARNED, FE_WARM_START_REFERENCE, FE_DIVERGENCE_REFERENCE, FE_DIVERGENCE_CONSISTENT,
-                   FE_BOUNDARY_CLAMP, FE_BOUNDARY_PARTICLES,
+                   FE_BOUNDARY_CLAMP, FE_BOUNDARY_PARTICLES, FE_RESIDUAL_COMPRESSION, FE_RESIDUAL_TWO_SIDED,
                    FIELDS, INT_FIELDS, LIB_PATH, FeOptions,
                    FeParams, FeTuneEntry, FeWall, FluidEngineError, check, load_library)
 from .collision import CollisionBox
@@ -28,7 +28,7 @@ __all__ = [
     "FE_WARM_START_REFERENCE", "FE_WARM_START_LEARNED", "FIELDS", "EXPORTED_SYMBOLS", "LIB_PATH",
     "FE_NET_DENSITY", "FE_NET_DIVERGENCE", "FE_MLP_INPUTS", "FE_MLP_HIDDEN", "FE_MLP_PARAMS", "FE_MLP_NORM",
     "Batch", "CollisionBox", "FeWall", "FE_DIVERGENCE_REFERENCE", "FE_DIVERGENCE_CONSISTENT",
-    "FE_BOUNDARY_CLAMP", "FE_BOUNDARY_PARTICLES",
+    "FE_BOUNDARY_CLAMP", "FE_BOUNDARY_PARTICLES", "FE_RESIDUAL_COMPRESSION", "FE_RESIDUAL_TWO_SIDED",
 ]
 
 
@@ -250,6 +250,19 @@ class Simulation:
 
     def reset_iteration_totals(self):
         check(self._lib.fe_reset_iteration_totals(self._ctx), "fe_reset_iteration_totals")
+
+    def solve_report_enable(self, enable: bool = True):
+        """Report each Jacobi solve's exit residual from now on (resets the averages)."""
+        check(self._lib.fe_solve_report_enable(self._ctx, 1 if enable else 0), "fe_solve_report_enable")
+
+    def solve_report(self, kind: int) -> dict:
+        """Exit residuals of the solves of `kind` (0 density, 1 divergence) since the last
+        enable, in units of rho0 / dt, each averaged over the solves."""
+        n = c_uint32()
+        out = (c_double * 4)()
+        check(self._lib.fe_solve_report(self._ctx, kind, byref(n), out), "fe_solve_report")
+        return {"solves": n.value, "mean_abs": out[0], "max_abs": out[1], "mean_expansion": out[2],
+                "mean_expansion_pressured": out[3]}
 
     def sync(self):
         check(self._lib.fe_sync(self._ctx), "fe_sync")

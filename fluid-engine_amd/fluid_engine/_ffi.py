@@ -25,6 +25,8 @@ FE_DIVERGENCE_REFERENCE = 0
 FE_DIVERGENCE_CONSISTENT = 1
 FE_BOUNDARY_CLAMP = 0
 FE_BOUNDARY_PARTICLES = 1
+FE_RESIDUAL_COMPRESSION = 0
+FE_RESIDUAL_TWO_SIDED = 1
 FE_NET_DENSITY = 0
 FE_NET_DIVERGENCE = 1
 FE_MLP_INPUTS = 12
@@ -95,7 +97,8 @@ class FeOptions(Structure):
         ("warm_start", c_uint32),
         ("divergence_order", c_uint32),
         ("boundary", c_uint32),
-        ("reserved", c_uint32 * 4),
+        ("residual", c_uint32),
+        ("reserved", c_uint32 * 3),
     ]
 
 
@@ -153,6 +156,8 @@ _SIGNATURES = [
     ("fe_stats", [c_void_p]),
     ("fe_solve_density", [c_void_p, POINTER(c_uint32)]),
     ("fe_solve_divergence", [c_void_p, POINTER(c_uint32)]),
+    ("fe_solve_report_enable", [c_void_p, c_int]),
+    ("fe_solve_report", [c_void_p, c_int, POINTER(c_uint32), POINTER(c_double)]),
     ("fe_step", [c_void_p, c_float]),
     ("fe_run", [c_void_p, c_float, c_uint32]),
     ("fe_read_stats", [c_void_p, POINTER(c_uint32)]),

@@ -102,6 +102,15 @@ typedef struct fe_params {
 #define FE_BOUNDARY_CLAMP 0u
 #define FE_BOUNDARY_PARTICLES 1u
 
+/* Residual of a Jacobi sweep that FE_SOLVER_CONVERGE compares with the tolerance (no
+ * reference counterpart). COMPRESSION: mean over particles of max(0, -(s - Ap)) dt / rho0, the
+ * DFSPH compression-only error (under-dense particles at the free surface are not errors).
+ * TWO_SIDED: the same, except that a particle whose pressure entering the sweep is > 0 counts
+ * |s - Ap| dt / rho0, so a start that over-predicts pressure is not taken as converged; a particle
+ * with zero pressure still counts only compression (the clamp p >= 0 cannot push it further). */
+#define FE_RESIDUAL_COMPRESSION 0u
+#define FE_RESIDUAL_TWO_SIDED 1u
+
 /* Run options. The first five mirror the runtime flags of `AppUI`
  * (src/renderer/ui.rs:12-49): sort_algorithm, use_cfl, use_solver_error_threshold,
  * density_error_pct, divergence_error_pct, whose host logic is in Renderer::step
@@ -119,7 +128,8 @@ typedef struct fe_options {
     uint32_t warm_start;                 /* FE_WARM_START_REFERENCE */
     uint32_t divergence_order;           /* FE_DIVERGENCE_REFERENCE */
     uint32_t boundary;                   /* FE_BOUNDARY_CLAMP */
-    uint32_t reserved[4];
+    uint32_t residual;                   /* FE_RESIDUAL_COMPRESSION */
+    uint32_t reserved[3];
 } fe_options;
 
 /* Device buffers of `GpuPhysicsData` (src/entities/particle.rs:137-161).
@@ -208,6 +218,16 @@ fe_status fe_stats(fe_ctx* ctx);                    /* fill 0 + stats.comp */
  * stream. */
 fe_status fe_solve_density(fe_ctx* ctx, uint32_t* iterations);
 fe_status fe_solve_divergence(fe_ctx* ctx, uint32_t* iterations);
+/* Exit residuals of the Jacobi solves (no reference counterpart; one scene). While enabled,
+ * each solve ends with a report of its last sweep's residual r = s - Ap (the residual the exit
+ * test saw), as the relative rate r dt / rho0 that the tolerances are stated in.
+ * fe_solve_report gives, for solve kind 0 (density) or 1 (divergence), the solves reported
+ * since the last enable and, averaged over them: out[0] mean |r|, out[1] the largest |r| of any
+ * particle, out[2] mean max(0, r) (the expansion part), out[3] the expansion part summed over
+ * the particles whose final pressure is > 0, over N. Enabling resets the counts; each report
+ * adds one gather and one reduction to the solve. */
+fe_status fe_solve_report_enable(fe_ctx* ctx, int enable);
+fe_status fe_solve_report(fe_ctx* ctx, int kind, uint32_t* solves, double out[4]);
 
 /* The physics frame of Renderer::step (src/renderer/mod.rs:227-364): frame-start neighbour
  * search and density+alpha, substeps while t < max_dt, then stats. With max_dt == dt it is

@@ -603,7 +603,9 @@ uint64_t pressure_update(State<R>& s) {
         if (wg) sum_ap += dot(ai, wall_terms(s, *wg, h, pi).g);
         R ap = dt * sum_ap;
         R err = src_i - ap;
-        resid += residual_q((float)err, (float)dt, s.p.target_density);
+        // FE_RESIDUAL_TWO_SIDED: a particle entering the sweep with pressure counts |s - Ap|
+        const R counted = s.o.residual == FE_RESIDUAL_TWO_SIDED && p_i > R(0) ? -std::fabs(err) : err;
+        resid += residual_q((float)counted, (float)dt, s.p.target_density);
         if (alpha_i > Lit<R>::eps6 && rho_i > Lit<R>::eps6) {
             R a_ii = -(dt / (rho_i * rho_i)) * alpha_i;
             if (std::fabs(a_ii) > Lit<R>::eps20) {

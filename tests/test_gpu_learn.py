@@ -298,8 +298,11 @@ def test_learned_warm_start_reduces_sweeps():
 def test_learned_warm_start_halves_sweeps_1m():
     """The 1,048,576-particle dam break: samples from frames 0-99 (2048 per solve, 204,800 per
     network), both networks trained on the GPU, then frames 100-499 with the learned start
-    against a second context with the reference start. The total Jacobi sweeps to the same
-    0.1 % / 1 % thresholds must drop at least 2x."""
+    against a second context with the reference start. Both stop on the two-sided residual
+    (FE_RESIDUAL_TWO_SIDED): a pressured particle counts |s - Ap|, so a start that over-predicts
+    pressure is not taken as converged. The total Jacobi sweeps to the same 0.1 % / 1 %
+    thresholds must drop at least 2x, and the exit residuals (the next 50 frames, reported)
+    must be no worse than the reference start's."""
     w = scene.dam_break_1m(seed=0)
     ref = fe.Simulation(w.params, w.positions, device=0, options=conv_opts())
     lrn = fe.Simulation(w.params, w.positions, device=0, options=conv_opts())
@@ -310,7 +313,12 @@ def test_learned_warm_start_halves_sweeps_1m():
         lrn.learn_collect(0)
         lrn.learn_train(fe.FE_NET_DENSITY, steps=2000, batch=4096, lr=2e-3, seed=1)
         lrn.learn_train(fe.FE_NET_DIVERGENCE, steps=2000, batch=4096, lr=2e-3, seed=2)
-        lrn.set_options(conv_opts(fe.FE_WARM_START_LEARNED))
+        ro = conv_opts()
+        ro.residual = fe.FE_RESIDUAL_TWO_SIDED
+        lo = conv_opts(fe.FE_WARM_START_LEARNED)
+        lo.residual = fe.FE_RESIDUAL_TWO_SIDED
+        ref.set_options(ro)
+        lrn.set_options(lo)
         ref.reset_iteration_totals()
         lrn.reset_iteration_totals()
         ref.run(400)
@@ -318,8 +326,20 @@ def test_learned_warm_start_halves_sweeps_1m():
         rd, rv, _ = ref.iteration_totals()
         ld, lv, _ = lrn.iteration_totals()
         gain = (rd + rv) / max(ld + lv, 1)
-        print(f"1M, frames 100-499: reference {rd} + {rv} sweeps, learned {ld} + {lv}, {gain:.2f}x fewer")
+        print(f"1M, frames 100-499, two-sided residual: reference {rd} + {rv} sweeps, learned {ld} + {lv}, "
+              f"{gain:.2f}x fewer")
         assert gain >= 2.0
+        for sim in (ref, lrn):
+            sim.solve_report_enable(True)
+            sim.run(50)
+        for kind, tol in ((0, 1e-3), (1, 1e-2)):
+            r, l = ref.solve_report(kind), lrn.solve_report(kind)
+            print(f"exit residuals, solve {kind}: reference {r}, learned {l}")
+            assert r["solves"] == l["solves"] == 50
+            # the pressured particles' expansion part is within the tolerance for both starts
+            assert l["mean_expansion_pressured"] <= tol and r["mean_expansion_pressured"] <= tol
+            # the learned start leaves no larger two-sided residual than the reference start
+            assert l["mean_abs"] <= 1.05 * r["mean_abs"]
         err_l = lrn.read_stats()[1] / w.n
         err_r = ref.read_stats()[1] / w.n
         assert err_l <= 1.5 * err_r + 5.0

@@ -290,6 +290,62 @@ def test_converge_mode_iterations_match(oracle_mod):
     assert frames == 3 and d >= 3 and v >= 3
 
 
+def test_converge_two_sided_residual_matches(oracle_mod):
+    """FE_RESIDUAL_TWO_SIDED (no reference counterpart): sweep counts, fields and the exit
+    reports' inputs equal the oracle's, frame by frame, and the two-sided test needs at least
+    as many sweeps as the compression-only one."""
+    w = scene.dam_break_32k(seed=2)
+    o = opts(mode=fe.FE_SOLVER_CONVERGE, maxit=100)
+    o.residual = fe.FE_RESIDUAL_TWO_SIDED
+    sim, orc = pair(w, o, oracle_mod)
+    one = fe.Simulation(w.params, w.positions, device=0, options=opts(mode=fe.FE_SOLVER_CONVERGE, maxit=100))
+    for frame in range(4):
+        sim.step()
+        orc.step()
+        one.step()
+        assert sim.last_iterations() == orc.last_iterations(), frame
+        for name in ("position_a", "velocity_a", "pressures"):
+            assert_exact(f"two-sided frame {frame} {name}", sim.download(name), orc.download(name))
+        if frame == 0:
+            d2, v2 = sim.last_iterations()
+            d1, v1 = one.last_iterations()
+            assert d2 >= d1 and v2 >= v1
+    one.close()
+
+
+def test_solve_report_closed_form(oracle_mod):
+    """fe_solve_report of a FIXED solve: its inputs are the last sweep's r = s - Ap, which the
+    oracle reproduces (one sweep from p = 0: Ap = 0 for the density solve, r = s)."""
+    w = scene.dam_break_32k(seed=4)
+    o = opts()
+    sim, orc = pair(w, o, oracle_mod)
+    sim.step()
+    orc.step()
+    sim.neighbor_search()
+    sim.density_alpha()
+    sim.viscosity()
+    sim.density_source()
+    s = sim.download("source_terms").astype(np.float64)
+    p = w.params
+    sim.solve_report_enable(True)
+    sim.set_params(scene.make_params(p.particle_radius, p.particle_mass, p.smoothing_radius, p.target_density,
+                                     p.viscosity, p.relax_factor, p.dt, 1, 1, tuple(p.gravity[:3]), tuple(p.box_min[:3]),
+                                     tuple(p.box_max[:3])))
+    sim.solve_density()
+    rep = sim.solve_report(0)
+    assert rep["solves"] == 1
+    # first sweep from p = 0: a = 0, so Ap = 0 and r = s exactly
+    scale = float(np.float32(p.dt)) / float(np.float32(p.target_density))
+    r = s * scale
+    assert abs(rep["mean_abs"] - np.abs(r).mean()) <= 1e-6 * max(np.abs(r).mean(), 1e-12) + 2 ** -24
+    assert abs(rep["mean_expansion"] - np.maximum(r, 0).mean()) <= 1e-6 * np.abs(r).mean() + 2 ** -24
+    assert abs(rep["max_abs"] - np.abs(r).max()) <= 2 ** -23 + 1e-6 * np.abs(r).max()
+    pres = sim.download("pressures")
+    want = (np.maximum(r, 0) * (pres > 0)).mean()
+    assert abs(rep["mean_expansion_pressured"] - want) <= 1e-6 * np.abs(r).mean() + 2 ** -24
+    sim.close()
+
+
 def test_host_flags_cfl_and_early_exit(oracle_mod):
     w = scene.rest_density_scene()
     o = opts(use_cfl=1, use_solver_error_threshold=1)

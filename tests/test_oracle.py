@@ -388,6 +388,40 @@ def test_pressure_update_diagonal_is_alpha(oracle_mod, small_block):
     assert abs(float(o.download("pressures")[k])) < 1e-4 * 2500.0
 
 
+def test_two_sided_residual_counts_pressured_expansion(oracle_mod, small_block):
+    """FE_RESIDUAL_TWO_SIDED (no reference counterpart): one particle with p_k = 2500 and
+    s = 0 everywhere. Its residual r_k = -Ap_k = -a_kk p_k is an expansion (r_k > 0), which the
+    compression-only residual ignores and the two-sided one counts as |r_k| dt / rho0; the
+    unpressured particles count the same in both. a_kk = -(dt / rho_k^2) alpha_k."""
+    w = small_block
+    dt = 1e-3
+    prm = scene.make_params(w.params.particle_radius, w.params.particle_mass, w.params.smoothing_radius,
+                            1000.0, 0.01, 1.0, dt, 1, 1, (0, -9.81, 0), (0, 0, 0), (1, 1, 1))
+    qs = []
+    for resid in (0, 1):
+        o = oracle_mod.Oracle(prm, w.positions, options=opts())
+        op = opts()
+        op.residual = resid
+        o.set_options(op)
+        o.neighbor_search()
+        o.density_alpha()
+        pb = o.download("position_b")
+        k = int(np.argmin(np.linalg.norm(pb[:, :3] - pb[:, :3].mean(axis=0), axis=1)))
+        p = np.zeros(w.n, F)
+        p[k] = 2500.0
+        o.upload("pressures", p)
+        o.upload("source_terms", np.zeros(w.n, F))
+        o.pressure_force()
+        qs.append(int(o.pressure_update()))
+        rho = float(o.download("densities")[k])
+        alpha = float(o.download("factors")[k])
+    a_kk = -(dt / (rho * rho)) * alpha
+    want = -a_kk * 2500.0 * dt / 1000.0 * 2 ** 32
+    extra = qs[1] - qs[0]
+    assert want > 0
+    assert abs(extra - want) <= 1e-4 * want, (extra, want)
+
+
 def test_integration_collision_closed_form(oracle_mod):
     pos = np.array([[0.5, 0.5, 0.5], [0.5, 0.005, 0.5], [0.999, 0.5, 0.5], [0.5, 0.5, -0.2]], F)
     prm = scene.make_params(0.01, 0.008, 0.04, 1000.0, 0.01, 0.5, 0.5, 1, 1, (0, -9.81, 0), (0, 0, 0), (1, 1, 1))
