@@ -47,9 +47,10 @@ ALGO_BYTES = {
 # features (48 B) and read the previous pressure (4 B).
 ALGO_BYTES["density_source_feat"] = ALGO_BYTES["density_source"] + 52.0
 ALGO_BYTES["divergence_source_feat"] = ALGO_BYTES["divergence_source"] + 52.0
-# density_alpha and divergence_source fused into one pass after the post-integrate search
-ALGO_BYTES["density_alpha_divergence_source"] = ALGO_BYTES["density_alpha"] + ALGO_BYTES["divergence_source"]
-ALGO_BYTES["density_alpha_divergence_source_feat"] = ALGO_BYTES["density_alpha"] + ALGO_BYTES["divergence_source_feat"]
+# density_alpha and divergence_source fused into one pass after the post-integrate search: the
+# position (12 B) both read is read once
+ALGO_BYTES["density_alpha_divergence_source"] = ALGO_BYTES["density_alpha"] + ALGO_BYTES["divergence_source"] - 12.0
+ALGO_BYTES["density_alpha_divergence_source_feat"] = ALGO_BYTES["density_alpha_divergence_source"] + 52.0
 # Warm-start inference: 2 (12 x 64 + 2 x 64 x 64 + 64) multiply-adds per particle on the matrix
 # cores (BF16 dense peak 2.5 PFLOP/s).
 MLP_FLOP = 2.0 * (12 * 64 + 2 * 64 * 64 + 64)

@@ -43,6 +43,10 @@ def autotune(sim, cache_path: str | None = None, reps: int = 20, retune: bool = 
     key = cache_key(sim.device_name(), sim.n, getattr(sim, "scenes", 1))
     cache = load_cache(cache_path)
     entry = cache["entries"].get(key)
+    # an entry written by a library with other tunable kernels would leave the missing ones
+    # at their defaults: measure again
+    if entry is not None and set(entry) != set(sim.tune_entries()):
+        entry = None
     if entry and not retune:
         try:
             for kernel, cfg in entry.items():

@@ -323,9 +323,14 @@ fe_status fe_set_wall(fe_ctx* ctx, const fe_wall* wall);
 fe_status fe_get_wall(const fe_ctx* ctx, fe_wall* out, int* attached);
 
 /* ---- launch-configuration autotuner (no reference counterpart) --------------------------------
- * Tunable kernels: the six list gathers (density_alpha, viscosity, density_source,
- * divergence_source, pressure_force, pressure_update: block 64..1024), build_list (block
- * 64, 128, 256) and the two integrations (block 64..1024 x 1, 2 or 4 particles per thread).
+ * Tunable kernels: the seven list gathers (block 64..1024), build_list (block 64, 128, 256)
+ * and the two integrations (block 64..1024 x 1, 2 or 4 particles per thread). In fe_step:
+ *   density_alpha                    the frame-start density pass (when the last search
+ *                                    cannot be reused) and fe_density_alpha
+ *   density_alpha_divergence_source  the post-integrate density + divergence-source pass
+ *   divergence_source                fe_divergence_source only
+ *   viscosity, density_source, pressure_force, pressure_update, build_list and the two
+ *   integrations                     every launch of that kernel. 
  * For the two Jacobi sweeps, items selects the variant: 1 gathers each listed pair from the
  * schedule-ordered mirrors; 2 stages each 64-particle window's neighbour union in LDS and
  * walks 16-bit lists (block 64, 128 or 256 = 1, 2 or 4 windows per block). fe_autotune

@@ -8,18 +8,22 @@ from fluid_engine._ffi import FluidEngineError
 
 
 class FakeSim:
-    def __init__(self, n=1000, scenes=1):
+    def __init__(self, n=1000, scenes=1, kernels=("pressure_force",)):
         self.n = n
         self.scenes = scenes
+        self.kernels = kernels
         self.measured = 0
         self.applied = {}
+
+    def tune_entries(self):
+        return {k: {"block": 256, "items": 1, "median_us": 0.0, "default_us": 0.0} for k in self.kernels}
 
     def device_name(self):
         return "gfx950/256cu"
 
     def autotune_now(self, reps):
         self.measured += 1
-        return {"pressure_force": {"block": 128, "items": 1, "median_us": 10.0, "default_us": 12.0}}
+        return {k: {"block": 128, "items": 1, "median_us": 10.0, "default_us": 12.0} for k in self.kernels}
 
     def set_tune(self, kernel, block, items):
         if block not in (64, 128, 256, 512, 1024):
@@ -66,3 +70,16 @@ def test_stale_entry_is_measured_again(tmp_path):
     r = tuning.autotune(s, path)
     assert s.measured == 1 and r["pressure_force"]["block"] == 128
     assert json.load(open(path))["entries"][key]["pressure_force"]["block"] == 128
+
+
+def test_entry_without_a_tunable_kernel_is_measured_again(tmp_path):
+    """A cached entry from a library with fewer tunable kernels would leave the new kernel at
+    its default: it is measured again, and the new entry names every kernel."""
+    path = str(tmp_path / "tune.json")
+    tuning.autotune(FakeSim(), path)
+    s = FakeSim(kernels=("pressure_force", "density_alpha_divergence_source"))
+    r = tuning.autotune(s, path)
+    assert s.measured == 1 and set(r) == {"pressure_force", "density_alpha_divergence_source"}
+    t = FakeSim(kernels=("pressure_force", "density_alpha_divergence_source"))
+    tuning.autotune(t, path)
+    assert t.measured == 0 and t.applied["density_alpha_divergence_source"] == (128, 1)
