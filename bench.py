@@ -407,9 +407,12 @@ def app_frame_measurement(fe, scene, with_cpu, warmup=20, frames=200):
     o.sort_algorithm = fe.FE_SORT_RADIX
     out = {"workload": f"{w.name}, moving wall {w.wall_wave[0]} m / {w.wall_wave[1]} Hz, fixed 4/4 sweeps",
            "frames": f"{warmup} warm-up + {frames} timed"}
-    for splat in (False, True):
+    # the reference records each frame into one command buffer; here the frame is replayed
+    # from a captured HIP graph (fe_use_graph), and also timed eagerly
+    for splat, graph in ((False, True), (True, True), (False, False)):
         sim = fe.Simulation(w.params, w.positions, device=0, options=o)
         sim.set_wall(fe.CollisionBox.from_params(w.params, *w.wall_wave))
+        sim.use_graph(graph)
         for _ in range(warmup):
             sim.step()
             if splat:
@@ -423,8 +426,9 @@ def app_frame_measurement(fe, scene, with_cpu, warmup=20, frames=200):
         ms = sim.timer_stop()
         wall = sim.wall()
         sim.close()
-        out["with_splat" if splat else "physics_only"] = {"frames_per_s": round(frames / (ms / 1e3), 2),
-                                                           "ms_per_frame": round(ms / frames, 4)}
+        key = ("with_splat" if splat else "physics_only") + ("" if graph else "_eager")
+        out[key] = {"frames_per_s": round(frames / (ms / 1e3), 2), "ms_per_frame": round(ms / frames, 4),
+                    "graph": graph}
         out["wall_min_x_after"] = round(float(wall.min[0]), 5)
     if with_cpu:
         # the oracle on the host cores: the same frames with the moving wall, physics only

@@ -87,6 +87,10 @@ void free_ctx(fe_ctx* c) {
     for (hipEvent_t e : c->tab_event)
         if (e) (void)hipEventDestroy(e);
     if (c->tab_ring) (void)hipHostFree(c->tab_ring);
+    for (hipEvent_t e : c->box_event)
+        if (e) (void)hipEventDestroy(e);
+    if (c->box_ring) (void)hipHostFree(c->box_ring);
+    if (c->box_dev) (void)hipFree(c->box_dev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -169,6 +173,29 @@ fe_status enqueue_scene_tab(fe_ctx* c) {
     if (s != FE_OK) return s;
     FE_HIP(hipMemcpyAsync(c->scene_tab, tab, per, hipMemcpyHostToDevice, c->stream), "scene table");
     FE_HIP(hipEventRecord(c->tab_event[slot], c->stream), "table event");
+    return FE_OK;
+}
+
+// One scene in graph mode: the frame's box copied to the device on the stream, from one of
+// kTabRing pinned host slots (the slot's previous copy, kTabRing frames ago, has finished).
+fe_status enqueue_live_box(fe_ctx* c) {
+    if (!c->box_ring) {
+        FE_HIP(hipMalloc((void**)&c->box_dev, 8 * sizeof(float)), "hipMalloc box");
+        FE_HIP(hipHostMalloc((void**)&c->box_ring, 8 * sizeof(float) * fe::kTabRing, hipHostMallocDefault),
+               "hipHostMalloc box");
+        for (hipEvent_t& e : c->box_event) FE_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), "box event");
+    }
+    const uint32_t slot = c->box_slot;
+    c->box_slot = (slot + 1) % fe::kTabRing;
+    FE_HIP(hipEventSynchronize(c->box_event[slot]), "box slot wait");
+    float* b = c->box_ring + 8 * (size_t)slot;
+    for (int a = 0; a < 3; a++) {
+        b[a] = c->params.box_min[a];
+        b[3 + a] = c->params.box_max[a];
+    }
+    b[6] = b[7] = 0.0f;
+    FE_HIP(hipMemcpyAsync(c->box_dev, b, 8 * sizeof(float), hipMemcpyHostToDevice, c->stream), "box copy");
+    FE_HIP(hipEventRecord(c->box_event[slot], c->stream), "box event");
     return FE_OK;
 }
 
@@ -453,7 +480,9 @@ fe_status wait_prev_stats(fe_ctx* c) {
     return FE_OK;
 }
 
-fe_status step(fe_ctx* c, float max_dt) {
+// stats_copy: read the frame's stats back to the host (a captured frame leaves that copy to
+// its replays).
+fe_status step(fe_ctx* c, float max_dt, bool stats_copy = true) {
     uint32_t dit = c->params.density_solver_iterations, vit = c->params.divergence_solver_iterations;
     if (c->options.use_cfl || c->options.use_solver_error_threshold) {
         fe_status s = wait_prev_stats(c);
@@ -488,7 +517,7 @@ fe_status step(fe_ctx* c, float max_dt) {
         if (!(c->params.dt > 0.0f)) break;   // a non-positive dt would never end the frame
     }
     FE_HIP(fe::launch_stats(c), "stats");
-    if (c->nscenes == 1) {
+    if (c->nscenes == 1 && stats_copy) {
         FE_HIP(hipMemcpyAsync(c->host_stats, c->stats, 12, hipMemcpyDeviceToHost, c->stream), "stats copy");
         FE_HIP(hipEventRecord(c->stats_event, c->stream), "stats event");
         c->stats_pending = true;
@@ -497,14 +526,17 @@ fe_status step(fe_ctx* c, float max_dt) {
     return FE_OK;
 }
 
-// A batch frame in FIXED mode from a captured graph: the steady-state frame (frame-start
-// search reused) is captured once and replayed; the first frame after creation, an upload or
-// a parameter change runs eagerly.
+// A frame in FIXED mode from a captured graph: the steady-state frame (frame-start search
+// reused) is captured once and replayed; the first frame after creation, an upload or a
+// parameter change runs eagerly. The capture holds kernels only; a single scene's stats
+// read-back is enqueued after each replay.
 fe_status step_graph(fe_ctx* c, float max_dt) {
     if (!(c->ns_fresh && c->da_fresh)) return step(c, max_dt);
+    if (c->graph_exec && c->graph_morton_bits != c->morton_bits) drop_graph(c);   // the box grew
     if (!c->graph_exec) {
+        c->graph_morton_bits = c->morton_bits;
         FE_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
-        fe_status st = step(c, max_dt);
+        fe_status st = step(c, max_dt, false);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (st != FE_OK) {
@@ -522,21 +554,44 @@ fe_status step_graph(fe_ctx* c, float max_dt) {
         c->frames--;
         for (int k = 0; k < 2; k++) c->total_iters[k] -= c->last_iters[k];
         for (int k = 0; k < 2; k++)
-            for (uint32_t sc = 0; sc < c->nscenes; sc++) c->scene_total_iters[k][sc] -= c->scene_iters[k][sc];
+            for (uint32_t sc = 0; c->nscenes > 1 && sc < c->nscenes; sc++)
+                c->scene_total_iters[k][sc] -= c->scene_iters[k][sc];
     }
     FE_HIP(hipGraphLaunch(c->graph_exec, c->stream), "graph launch");
+    if (c->nscenes == 1) {
+        FE_HIP(hipMemcpyAsync(c->host_stats, c->stats, 12, hipMemcpyDeviceToHost, c->stream), "stats copy");
+        FE_HIP(hipEventRecord(c->stats_event, c->stream), "stats event");
+        c->stats_pending = true;
+    }
     c->frames++;
     for (int k = 0; k < 2; k++) c->total_iters[k] += c->last_iters[k];
     for (int k = 0; k < 2; k++)
-        for (uint32_t sc = 0; sc < c->nscenes; sc++) c->scene_total_iters[k][sc] += c->scene_iters[k][sc];
+        for (uint32_t sc = 0; c->nscenes > 1 && sc < c->nscenes; sc++) c->scene_total_iters[k][sc] += c->scene_iters[k][sc];
     return FE_OK;
+}
+
+// Whether a single scene's frame can be replayed from a graph: fixed sweeps and nothing that
+// makes the host decide inside the frame (frame control, learning, exit reports) or changes
+// kernel arguments from frame to frame other than the box (wall samples follow the box).
+bool single_graph_ok(const fe_ctx* c) {
+    const fe_options& o = c->options;
+    return !o.use_cfl && !o.use_solver_error_threshold && o.warm_start == FE_WARM_START_REFERENCE &&
+           o.boundary == FE_BOUNDARY_CLAMP && !c->report_on && !fe::learn_features_wanted(c, 0) &&
+           !fe::learn_features_wanted(c, 1);
 }
 
 fe_status run_frame(fe_ctx* c, float max_dt) {
     fe_status s = apply_walls(c);
     if (s != FE_OK) return s;
-    const bool graph = c->batch_graph && c->nscenes > 1 && c->options.solver_mode == FE_SOLVER_FIXED && !c->profiling;
+    const bool graph = c->batch_graph && c->options.solver_mode == FE_SOLVER_FIXED && !c->profiling &&
+                       (c->nscenes > 1 || single_graph_ok(c));
+    if (graph && c->nscenes == 1) {
+        // every frame of a graph-mode scene takes its box from the device copy, eager or replayed
+        if ((s = enqueue_live_box(c)) != FE_OK) return s;
+        c->live_box = true;
+    }
     s = graph ? step_graph(c, max_dt) : step(c, max_dt);
+    c->live_box = false;
     if (s == FE_OK) advance_walls(c, max_dt);
     return s;
 }
@@ -1443,6 +1498,14 @@ fe_status fe_batch_get_wall(const fe_ctx* c, uint32_t scene, fe_wall* out, int* 
 fe_status fe_batch_use_graph(fe_ctx* c, int enable) {
     FE_CHECK_CTX(c);
     if (c->nscenes <= 1) return fail(FE_ERR_INVALID_ARGUMENT, "not a batch context (fe_batch_create)");
+    DeviceGuard g(c->device);
+    drop_graph(c);
+    c->batch_graph = enable != 0;
+    return FE_OK;
+}
+
+fe_status fe_use_graph(fe_ctx* c, int enable) {
+    FE_CHECK_CTX(c);
     DeviceGuard g(c->device);
     drop_graph(c);
     c->batch_graph = enable != 0;

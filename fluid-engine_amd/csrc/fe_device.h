@@ -40,6 +40,9 @@ struct Scenes {
     const Sph* tab;
     const uint8_t* sid;   // scene of each particle index (the same in every particle order)
     uint32_t count;
+    // one scene replayed from a captured frame: the box (min xyz, max xyz) of the current
+    // frame, copied to the device before each replay (a moving wall changes it every frame)
+    const float* box = nullptr;
 };
 
 __device__ __forceinline__ Sph scene_at(const Scenes& sc, const Sph& s0, uint32_t idx) {
@@ -84,7 +87,7 @@ inline Sph sph_from(const fe_params& p, uint32_t n, uint32_t T, uint32_t tab0, u
 
 inline Sph sph_of(const fe_ctx* c) { return sph_from(c->params, c->n, c->T, 0u, 0u, c->exact_fast); }
 
-inline Scenes scenes_of(const fe_ctx* c) { return Scenes{c->scene_tab, c->sid, c->nscenes}; }
+inline Scenes scenes_of(const fe_ctx* c) { return Scenes{c->scene_tab, c->sid, c->nscenes, c->live_box ? c->box_dev : nullptr}; }
 
 __device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, uint32_t mask) {
     return ((uint32_t)cx * kP1 ^ (uint32_t)cy * kP2 ^ (uint32_t)cz * kP3) & mask;

@@ -171,6 +171,15 @@ struct fe_ctx {
     bool batch_graph = false;                   // replay the frame from a captured graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // one scene in graph mode: the frame's box is read from box_dev (copied before every
+    // frame from a ring of pinned host slots) while live_box is set; the capture holds the
+    // Morton key width it was made with
+    float* box_dev = nullptr;
+    float* box_ring = nullptr;                  // pinned, kTabRing x 8
+    hipEvent_t box_event[fe::kTabRing] = {};
+    uint32_t box_slot = 0;
+    bool live_box = false;
+    uint32_t graph_morton_bits = 0;
 
     // Moving walls (one per scene) applied at the start of every frame. A batch whose walls
     // move re-copies its scene constants each frame from a ring of pinned host tables.

@@ -141,7 +141,8 @@ __device__ __forceinline__ void pressure_integration_one(uint32_t i, const Sph& 
                                                          float4* __restrict__ pos_a) {
     const Sph s = scene_at(sc, s0, i);
     const float dt = s.dt, rad = s.rad;
-    const float3 mn = make_float3(s.mnx, s.mny, s.mnz), mx = make_float3(s.mxx, s.mxy, s.mxz);
+    const float3 mn = sc.box ? make_float3(sc.box[0], sc.box[1], sc.box[2]) : make_float3(s.mnx, s.mny, s.mnz);
+    const float3 mx = sc.box ? make_float3(sc.box[3], sc.box[4], sc.box[5]) : make_float3(s.mxx, s.mxy, s.mxz);
     const float4 p = pos_b[i], v = vel[i], a = pacc[i];
     float vx = v.x + a.x * dt, vy = v.y + a.y * dt, vz = v.z + a.z * dt;
     float px = p.x + vx * dt, py = p.y + vy * dt, pz = p.z + vz * dt;

@@ -222,6 +222,10 @@ class Simulation:
         from the wall, then advances the wall by the frame's max_dt."""
         check(self._lib.fe_set_wall(self._ctx, None if box is None else byref(box.wall())), "fe_set_wall")
 
+    def use_graph(self, enable: bool = True):
+        """Replay the steady-state FIXED frame from a captured HIP graph (fe_use_graph)."""
+        check(self._lib.fe_use_graph(self._ctx, 1 if enable else 0), "fe_use_graph")
+
     def wall(self) -> CollisionBox | None:
         """The attached wall as advanced so far, or None."""
         w = FeWall()

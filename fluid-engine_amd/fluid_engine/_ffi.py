@@ -201,6 +201,7 @@ _SIGNATURES = [
     ("fe_batch_iterations", [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint64),
                              POINTER(c_uint64)]),
     ("fe_batch_use_graph", [c_void_p, c_int]),
+    ("fe_use_graph", [c_void_p, c_int]),
     ("fe_batch_set_wall", [c_void_p, c_uint32, POINTER(FeWall)]),
     ("fe_batch_get_wall", [c_void_p, c_uint32, POINTER(FeWall), POINTER(c_int)]),
     ("fe_autotune", [c_void_p, c_uint32]),

@@ -394,6 +394,13 @@ fe_status fe_batch_iterations(fe_ctx* ctx, uint32_t scene, uint32_t* density, ui
 /* With FE_SOLVER_FIXED, capture the steady-state frame of the batch once in a HIP graph and
  * replay it for every later frame (parameter, option and field changes drop the graph). */
 fe_status fe_batch_use_graph(fe_ctx* ctx, int enable);
+/* The same for any context (no reference counterpart; the reference records each physics
+ * frame into one command buffer, src/renderer/mod.rs:279-363). A single scene replays its
+ * steady-state frame when it runs FE_SOLVER_FIXED with no CFL or early-exit flags, the
+ * reference warm start, the clamp walls and no learning or exit reports; other frames run
+ * eagerly. An attached moving wall keeps moving: the frame's box is copied to the device
+ * before each frame. Results equal the eager frames bit for bit. */
+fe_status fe_use_graph(fe_ctx* ctx, int enable);
 /* A moving wall per scene, as fe_set_wall / fe_get_wall: the scene's box comes from its
  * wall at the start of every frame (the per-scene constants are copied to the device on the
  * context's stream, also before graph replays). */
