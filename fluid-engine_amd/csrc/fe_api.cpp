@@ -471,11 +471,32 @@ fe_status neighbor_search(fe_ctx* c) {
     return FE_OK;
 }
 
+// The frame's stats (and, with the watchdog, its health words) read back behind the frame.
+fe_status enqueue_stats_copy(fe_ctx* c) {
+    FE_HIP(hipMemcpyAsync(c->host_stats, c->stats, c->watchdog ? 24 : 12, hipMemcpyDeviceToHost, c->stream),
+           "stats copy");
+    FE_HIP(hipEventRecord(c->stats_event, c->stream), "stats event");
+    c->stats_pending = true;
+    c->stats_frame = c->frames;
+    c->stats_health = c->watchdog;
+    return FE_OK;
+}
+
 fe_status wait_prev_stats(fe_ctx* c) {
     if (c->stats_pending) {
         FE_HIP(hipEventSynchronize(c->stats_event), "stats wait");
         std::memcpy(c->prev_stats, c->host_stats, sizeof(c->prev_stats));
         c->stats_pending = false;
+        if (c->stats_health) {
+            fe_health& h = c->health;
+            h.frame = c->stats_frame;
+            h.nonfinite = c->host_stats[4];
+            h.first_nonfinite = c->host_stats[5];
+            std::memcpy(&h.max_speed, &c->host_stats[0], sizeof(float));
+            h.speed_limit = c->speed_limit;
+            h.flags = (h.nonfinite ? FE_HEALTH_NONFINITE : 0u) |
+                      (c->speed_limit > 0.0f && !(h.max_speed <= c->speed_limit) ? FE_HEALTH_SPEED : 0u);
+        }
     }
     return FE_OK;
 }
@@ -484,6 +505,19 @@ fe_status wait_prev_stats(fe_ctx* c) {
 // its replays).
 fe_status step(fe_ctx* c, float max_dt, bool stats_copy = true) {
     uint32_t dit = c->params.density_solver_iterations, vit = c->params.divergence_solver_iterations;
+    if (c->watchdog && c->watchdog_halt) {
+        fe_status s0 = wait_prev_stats(c);
+        if (s0 != FE_OK) return s0;
+        const fe_health& h = c->health;
+        if (h.flags & FE_HEALTH_NONFINITE)
+            return fail(FE_ERR_UNSTABLE, "watchdog: frame " + std::to_string(h.frame) + " left " +
+                                             std::to_string(h.nonfinite) + " particles with a non-finite field (first: " +
+                                             std::to_string(h.first_nonfinite) + ")");
+        if (h.flags & FE_HEALTH_SPEED)
+            return fail(FE_ERR_UNSTABLE, "watchdog: frame " + std::to_string(h.frame) + " max speed " +
+                                             std::to_string(h.max_speed) + " exceeds the limit " +
+                                             std::to_string(h.speed_limit));
+    }
     if (c->options.use_cfl || c->options.use_solver_error_threshold) {
         fe_status s = wait_prev_stats(c);
         if (s != FE_OK) return s;
@@ -517,12 +551,11 @@ fe_status step(fe_ctx* c, float max_dt, bool stats_copy = true) {
         if (!(c->params.dt > 0.0f)) break;   // a non-positive dt would never end the frame
     }
     FE_HIP(fe::launch_stats(c), "stats");
-    if (c->nscenes == 1 && stats_copy) {
-        FE_HIP(hipMemcpyAsync(c->host_stats, c->stats, 12, hipMemcpyDeviceToHost, c->stream), "stats copy");
-        FE_HIP(hipEventRecord(c->stats_event, c->stream), "stats event");
-        c->stats_pending = true;
-    }
     c->frames++;
+    if (c->nscenes == 1 && stats_copy) {
+        fe_status s2 = enqueue_stats_copy(c);
+        if (s2 != FE_OK) return s2;
+    }
     return FE_OK;
 }
 
@@ -558,12 +591,11 @@ fe_status step_graph(fe_ctx* c, float max_dt) {
                 c->scene_total_iters[k][sc] -= c->scene_iters[k][sc];
     }
     FE_HIP(hipGraphLaunch(c->graph_exec, c->stream), "graph launch");
-    if (c->nscenes == 1) {
-        FE_HIP(hipMemcpyAsync(c->host_stats, c->stats, 12, hipMemcpyDeviceToHost, c->stream), "stats copy");
-        FE_HIP(hipEventRecord(c->stats_event, c->stream), "stats event");
-        c->stats_pending = true;
-    }
     c->frames++;
+    if (c->nscenes == 1) {
+        fe_status s2 = enqueue_stats_copy(c);
+        if (s2 != FE_OK) return s2;
+    }
     for (int k = 0; k < 2; k++) c->total_iters[k] += c->last_iters[k];
     for (int k = 0; k < 2; k++)
         for (uint32_t sc = 0; c->nscenes > 1 && sc < c->nscenes; sc++) c->scene_total_iters[k][sc] += c->scene_iters[k][sc];
@@ -656,9 +688,9 @@ static fe_status alloc_ctx(fe_ctx* c, const float* xyz) {
     ALLOC(c->dens, s1); ALLOC(c->fac, s1); ALLOC(c->src, s1); ALLOC(c->pres, s1);
     ALLOC(c->keys[0], tb); ALLOC(c->keys[1], tb); ALLOC(c->vals[0], tb); ALLOC(c->vals[1], tb);
     ALLOC(c->range, 2 * tb);
-    ALLOC(c->stats, 16);
+    ALLOC(c->stats, 32);
     ALLOC(c->resid, sizeof(unsigned long long) * slots);
-    ALLOC(c->stats_partial, sizeof(uint32_t) * 3 * fe::kStatsBlocks);
+    ALLOC(c->stats_partial, sizeof(uint32_t) * 5 * fe::kStatsBlocks);
     ALLOC(c->nbr, sizeof(uint32_t) * fe::kMaxNbr * (((size_t)n + 15) / 16 * 16));   // whole 16-item tiles
     ALLOC(c->nbr_count, sizeof(uint32_t) * (size_t)n);
     for (int k = 0; k < 2; k++) {
@@ -676,7 +708,7 @@ static fe_status alloc_ctx(fe_ctx* c, const float* xyz) {
 #undef ALLOC
     e = fe::sort_scratch_alloc(&c->sort, c->T, c->stream);
     if (e != hipSuccess) return hip_fail(e, "sort scratch");
-    e = hipHostMalloc((void**)&c->host_stats, 16, hipHostMallocDefault);
+    e = hipHostMalloc((void**)&c->host_stats, 32, hipHostMallocDefault);
     if (e == hipSuccess)
         e = hipHostMalloc((void**)&c->host_resid, sizeof(unsigned long long) * slots, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->stats_event, hipEventDisableTiming);
@@ -697,14 +729,14 @@ static fe_status alloc_ctx(fe_ctx* c, const float* xyz) {
         if (e == hipSuccess) e = hipMemset(z, 0, v4);
     for (void* z : {(void*)c->dens, (void*)c->fac, (void*)c->src, (void*)c->pres})
         if (e == hipSuccess) e = hipMemset(z, 0, s1);
-    if (e == hipSuccess) e = hipMemset(c->stats, 0, 16);
+    if (e == hipSuccess) e = hipMemset(c->stats, 0, 32);
     if (e == hipSuccess) e = hipMemset(c->range, 0xFF, 2 * tb);
     if (e == hipSuccess) e = hipMemset(c->keys[0], 0xFF, tb);
     if (e == hipSuccess) e = hipMemset(c->vals[0], 0xFF, tb);
     if (e == hipSuccess) e = fe::launch_iota(c, c->sched[0], n);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) return hip_fail(e, "initial upload");
-    std::memset(c->host_stats, 0, 16);
+    std::memset(c->host_stats, 0, 32);
     return FE_OK;
 }
 
@@ -755,7 +787,7 @@ fe_status fe_create(const fe_params* params, const float* xyz, uint32_t n, int d
         free_ctx(c);
         return st;
     }
-    std::memset(c->host_stats, 0, 16);
+    std::memset(c->host_stats, 0, 32);
     *out = c;
     return FE_OK;
 }
@@ -1509,6 +1541,35 @@ fe_status fe_use_graph(fe_ctx* c, int enable) {
     DeviceGuard g(c->device);
     drop_graph(c);
     c->batch_graph = enable != 0;
+    return FE_OK;
+}
+
+fe_status fe_set_watchdog(fe_ctx* c, int enable, float speed_limit, int halt) {
+    FE_CHECK_CTX(c);
+    if (enable && c->nscenes > 1) return fail(FE_ERR_UNSUPPORTED, "the watchdog takes a single-scene context");
+    if (!(speed_limit >= 0.0f)) return fail(FE_ERR_INVALID_ARGUMENT, "speed_limit must be >= 0 (0: no limit)");
+    DeviceGuard g(c->device);
+    FE_HIP(hipStreamSynchronize(c->stream), "watchdog wait");
+    fe_status s = wait_prev_stats(c);
+    if (s != FE_OK) return s;
+    drop_graph(c);   // the stats kernels differ
+    c->watchdog = enable != 0;
+    c->watchdog_halt = enable && halt;
+    c->speed_limit = speed_limit;
+    c->health = fe_health{};
+    c->health.first_nonfinite = 0xFFFFFFFFu;
+    c->health.speed_limit = speed_limit;
+    return FE_OK;
+}
+
+fe_status fe_get_health(fe_ctx* c, fe_health* out) {
+    FE_CHECK_CTX(c);
+    if (!out) return fail(FE_ERR_INVALID_ARGUMENT, "null out");
+    if (!c->watchdog) return fail(FE_ERR_INVALID_ARGUMENT, "the watchdog is off (fe_set_watchdog)");
+    DeviceGuard g(c->device);
+    fe_status s = wait_prev_stats(c);
+    if (s != FE_OK) return s;
+    *out = c->health;
     return FE_OK;
 }
 

@@ -286,6 +286,12 @@ struct fe_ctx {
     bool stats_pending = false;
     bool have_prev_stats = false;
     uint32_t prev_stats[3] = {0, 0, 0};
+    uint64_t stats_frame = 0;      // frames run when the pending stats were enqueued
+    bool stats_health = false;     // the pending copy holds the health words
+    // the watchdog (fe_set_watchdog): health words in stats[4..5], the last report
+    bool watchdog = false, watchdog_halt = false;
+    float speed_limit = 0.0f;
+    fe_health health{};
 
     // exit reports of the Jacobi solves (fe_solve_report_enable): r = s - Ap by schedule
     // position, and per solve kind the sums of k_residual_reduce

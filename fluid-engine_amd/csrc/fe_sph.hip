@@ -283,29 +283,50 @@ __device__ __forceinline__ uint32_t to_u32_sat(float x) {
 // [max speed bits, density error sum, divergence error sum], then one block combines them
 // (which replaces the fill-0 of stats_pipeline.rs:56-66). Max and wrapping u32 sums are
 // order-independent, so the result equals the reference's atomics bit for bit.
+// With kHealth (the watchdog, no reference counterpart) the pass also counts the particles
+// whose position_a, velocity_a, density, source term or pressure is not finite and keeps the
+// lowest such index: partials [3 blocks + b] (count) and [4 blocks + b] (index).
+template <bool kHealth>
 __global__ __launch_bounds__(kBlock) void k_stats(uint32_t n, const float4* __restrict__ vel,
                                                   const float* __restrict__ dens, const float* __restrict__ src,
-                                                  float rho0, uint32_t* __restrict__ partial) {
-    __shared__ uint32_t part[3][kBlock / 64];
-    uint32_t mx = 0, s1 = 0, s2 = 0;
+                                                  float rho0, uint32_t* __restrict__ partial,
+                                                  const float4* __restrict__ pos, const float* __restrict__ pres) {
+    __shared__ uint32_t part[5][kBlock / 64];
+    uint32_t mx = 0, s1 = 0, s2 = 0, bad = 0, first = 0xFFFFFFFFu;
     for (uint32_t i = gid(); i < n; i += gridDim.x * kBlock) {
         const float4 v = vel[i];
         const float speed = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
         mx = max(mx, __float_as_uint(speed));
-        s1 += min(to_u32_sat(fabsf(dens[i] - rho0) * 1.0f), 10000u);
-        s2 += min(to_u32_sat(fabsf(src[i]) * 10.0f), 10000u);
+        const float d = dens[i], sr = src[i];
+        s1 += min(to_u32_sat(fabsf(d - rho0) * 1.0f), 10000u);
+        s2 += min(to_u32_sat(fabsf(sr) * 10.0f), 10000u);
+        if constexpr (kHealth) {
+            const float4 p = pos[i];
+            const bool fin = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(v.x) && isfinite(v.y) &&
+                             isfinite(v.z) && isfinite(d) && isfinite(sr) && isfinite(pres[i]);
+            if (!fin) {
+                bad++;
+                first = min(first, i);
+            }
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
         s1 += (uint32_t)__shfl_xor((int)s1, off, 64);
         s2 += (uint32_t)__shfl_xor((int)s2, off, 64);
+        if constexpr (kHealth) {
+            bad += (uint32_t)__shfl_xor((int)bad, off, 64);
+            first = min(first, (uint32_t)__shfl_xor((int)first, off, 64));
+        }
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) {
         part[0][w] = mx;
         part[1][w] = s1;
         part[2][w] = s2;
+        part[3][w] = bad;
+        part[4][w] = first;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -313,33 +334,50 @@ __global__ __launch_bounds__(kBlock) void k_stats(uint32_t n, const float4* __re
             mx = max(mx, part[0][k]);
             s1 += part[1][k];
             s2 += part[2][k];
+            bad += part[3][k];
+            first = min(first, part[4][k]);
         }
         partial[blockIdx.x] = mx;
         partial[gridDim.x + blockIdx.x] = s1;
         partial[2 * gridDim.x + blockIdx.x] = s2;
+        if constexpr (kHealth) {
+            partial[3 * gridDim.x + blockIdx.x] = bad;
+            partial[4 * gridDim.x + blockIdx.x] = first;
+        }
     }
 }
 
+template <bool kHealth>
 __global__ __launch_bounds__(kBlock) void k_stats_final(const uint32_t* __restrict__ partial, uint32_t blocks,
                                                         uint32_t* __restrict__ stats) {
-    __shared__ uint32_t part[3][kBlock / 64];
-    uint32_t mx = 0, s1 = 0, s2 = 0;
+    __shared__ uint32_t part[5][kBlock / 64];
+    uint32_t mx = 0, s1 = 0, s2 = 0, bad = 0, first = 0xFFFFFFFFu;
     for (uint32_t k = threadIdx.x; k < blocks; k += kBlock) {
         mx = max(mx, partial[k]);
         s1 += partial[blocks + k];
         s2 += partial[2 * blocks + k];
+        if constexpr (kHealth) {
+            bad += partial[3 * blocks + k];
+            first = min(first, partial[4 * blocks + k]);
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
         s1 += (uint32_t)__shfl_xor((int)s1, off, 64);
         s2 += (uint32_t)__shfl_xor((int)s2, off, 64);
+        if constexpr (kHealth) {
+            bad += (uint32_t)__shfl_xor((int)bad, off, 64);
+            first = min(first, (uint32_t)__shfl_xor((int)first, off, 64));
+        }
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) {
         part[0][w] = mx;
         part[1][w] = s1;
         part[2][w] = s2;
+        part[3][w] = bad;
+        part[4][w] = first;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -347,6 +385,12 @@ __global__ __launch_bounds__(kBlock) void k_stats_final(const uint32_t* __restri
             mx = max(mx, part[0][k]);
             s1 += part[1][k];
             s2 += part[2][k];
+            bad += part[3][k];
+            first = min(first, part[4][k]);
+        }
+        if constexpr (kHealth) {
+            stats[4] = bad;
+            stats[5] = first;
         }
         stats[0] = mx;
         stats[1] = s1;
@@ -677,11 +721,19 @@ hipError_t launch_stats(fe_ctx* c) {
     if (blocks > kStatsBlocks) blocks = kStatsBlocks;
     {
         Launch L(c, "stats");
-        hipLaunchKernelGGL(k_stats, dim3(blocks), dim3(kBlock), 0, c->stream, c->n, c->vel_a, c->dens, c->src,
-                           c->params.target_density, c->stats_partial);
+        if (c->watchdog)
+            hipLaunchKernelGGL(k_stats<true>, dim3(blocks), dim3(kBlock), 0, c->stream, c->n, c->vel_a, c->dens, c->src,
+                               c->params.target_density, c->stats_partial, c->pos_a, c->pres);
+        else
+            hipLaunchKernelGGL(k_stats<false>, dim3(blocks), dim3(kBlock), 0, c->stream, c->n, c->vel_a, c->dens,
+                               c->src, c->params.target_density, c->stats_partial, c->pos_a, c->pres);
     }
     Launch L(c, "stats_final");
-    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(kBlock), 0, c->stream, c->stats_partial, blocks, c->stats);
+    if (c->watchdog)
+        hipLaunchKernelGGL(k_stats_final<true>, dim3(1), dim3(kBlock), 0, c->stream, c->stats_partial, blocks, c->stats);
+    else
+        hipLaunchKernelGGL(k_stats_final<false>, dim3(1), dim3(kBlock), 0, c->stream, c->stats_partial, blocks,
+                           c->stats);
     return hipGetLastError();
 }
 

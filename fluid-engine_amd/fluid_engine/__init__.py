@@ -18,7 +18,8 @@ from ._ffi import (EXPORTED_SYMBOLS, FE_MLP_HIDDEN, FE_MLP_INPUTS, FE_MLP_NORM, 
                    FE_NET_DIVERGENCE, FE_SOLVER_CONVERGE, FE_SOLVER_FIXED, FE_SORT_BITONIC, FE_SORT_RADIX,
                    FE_WARM_START_LEARNED, FE_WARM_START_REFERENCE, FE_DIVERGENCE_REFERENCE, FE_DIVERGENCE_CONSISTENT,
                    FE_BOUNDARY_CLAMP, FE_BOUNDARY_PARTICLES, FE_RESIDUAL_COMPRESSION, FE_RESIDUAL_TWO_SIDED,
-                   FIELDS, INT_FIELDS, LIB_PATH, FeOptions,
+                   FE_ERR_UNSTABLE, FE_HEALTH_NONFINITE, FE_HEALTH_SPEED,
+                   FIELDS, INT_FIELDS, LIB_PATH, FeHealth, FeOptions,
                    FeParams, FeTuneEntry, FeWall, FluidEngineError, check, load_library)
 from .collision import CollisionBox
 
@@ -221,6 +222,20 @@ class Simulation:
         """Attach a copy of `box` (None detaches): every later frame takes the parameter box
         from the wall, then advances the wall by the frame's max_dt."""
         check(self._lib.fe_set_wall(self._ctx, None if box is None else byref(box.wall())), "fe_set_wall")
+
+    def set_watchdog(self, enable: bool = True, speed_limit: float = 0.0, halt: bool = False):
+        """Count non-finite particles (and the max speed against speed_limit) in every frame's
+        stats pass; with halt, the next frame refuses to run (FE_ERR_UNSTABLE) after an
+        unhealthy one."""
+        check(self._lib.fe_set_watchdog(self._ctx, 1 if enable else 0, speed_limit, 1 if halt else 0),
+              "fe_set_watchdog")
+
+    def health(self) -> dict:
+        """The watchdog's report of the last frame."""
+        h = FeHealth()
+        check(self._lib.fe_get_health(self._ctx, byref(h)), "fe_get_health")
+        return {"frame": h.frame, "nonfinite": h.nonfinite, "first_nonfinite": h.first_nonfinite,
+                "max_speed": h.max_speed, "speed_limit": h.speed_limit, "flags": h.flags}
 
     def use_graph(self, enable: bool = True):
         """Replay the steady-state FIXED frame from a captured HIP graph (fe_use_graph)."""

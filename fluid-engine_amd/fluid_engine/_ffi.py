@@ -15,6 +15,9 @@ LIB_NAME = "libfluid_engine.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 FE_OK = 0
+FE_ERR_UNSTABLE = 6
+FE_HEALTH_NONFINITE = 1
+FE_HEALTH_SPEED = 2
 FE_SORT_BITONIC = 0
 FE_SORT_RADIX = 1
 FE_SOLVER_FIXED = 0
@@ -115,8 +118,23 @@ class FeWall(Structure):
     ]
 
 
+class FeHealth(Structure):
+    """fe_health: the watchdog's report of the last frame."""
+
+    _fields_ = [
+        ("frame", c_uint64),
+        ("nonfinite", c_uint32),
+        ("first_nonfinite", c_uint32),
+        ("max_speed", c_float),
+        ("speed_limit", c_float),
+        ("flags", c_uint32),
+        ("reserved", c_uint32),
+    ]
+
+
 assert ctypes.sizeof(FeParams) == 112
 assert ctypes.sizeof(FeWall) == 40
+assert ctypes.sizeof(FeHealth) == 32
 
 
 class FluidEngineError(RuntimeError):
@@ -202,6 +220,8 @@ _SIGNATURES = [
                              POINTER(c_uint64)]),
     ("fe_batch_use_graph", [c_void_p, c_int]),
     ("fe_use_graph", [c_void_p, c_int]),
+    ("fe_set_watchdog", [c_void_p, c_int, c_float, c_int]),
+    ("fe_get_health", [c_void_p, POINTER(FeHealth)]),
     ("fe_batch_set_wall", [c_void_p, c_uint32, POINTER(FeWall)]),
     ("fe_batch_get_wall", [c_void_p, c_uint32, POINTER(FeWall), POINTER(c_int)]),
     ("fe_autotune", [c_void_p, c_uint32]),

@@ -36,6 +36,7 @@ typedef int32_t fe_status;
 #define FE_ERR_DEVICE 3
 #define FE_ERR_UNSUPPORTED 4
 #define FE_ERR_SIZE_MISMATCH 5
+#define FE_ERR_UNSTABLE 6   /* the watchdog halted a frame (fe_set_watchdog) */
 
 /* ---- parameter block ---------------------------------------------------------------------
  * Byte layout of `SimulationParams` (src/entities/particle.rs:310-331) and of the std140
@@ -401,6 +402,28 @@ fe_status fe_batch_use_graph(fe_ctx* ctx, int enable);
  * eagerly. An attached moving wall keeps moving: the frame's box is copied to the device
  * before each frame. Results equal the eager frames bit for bit. */
 fe_status fe_use_graph(fe_ctx* ctx, int enable);
+
+/* ---- watchdog (no reference counterpart) ------------------------------------------------------
+ * The reference's only health signal is the max-speed atomic of stats.comp:22-23. With the
+ * watchdog on, the stats pass of every frame also counts the particles whose position_a,
+ * velocity_a, density, source term or pressure is not finite, and keeps the lowest such index
+ * (hash order). fe_get_health reports the last frame (it waits for that frame). With halt set,
+ * fe_step / fe_run check the previous frame before running one and return FE_ERR_UNSTABLE
+ * when it left a non-finite particle, or a max speed above speed_limit (speed_limit > 0; 0
+ * sets no limit). Single-scene contexts; results are unchanged. */
+#define FE_HEALTH_NONFINITE 1u
+#define FE_HEALTH_SPEED 2u
+typedef struct fe_health {
+    uint64_t frame;            /* frames run when the reported frame ended (0: none yet) */
+    uint32_t nonfinite;        /* particles with a non-finite field */
+    uint32_t first_nonfinite;  /* the lowest such index (hash order), 0xFFFFFFFF if none */
+    float max_speed;           /* the frame's max |velocity_a| (the stats buffer's word 0) */
+    float speed_limit;
+    uint32_t flags;            /* FE_HEALTH_NONFINITE | FE_HEALTH_SPEED */
+    uint32_t reserved;
+} fe_health;
+fe_status fe_set_watchdog(fe_ctx* ctx, int enable, float speed_limit, int halt);
+fe_status fe_get_health(fe_ctx* ctx, fe_health* out);
 /* A moving wall per scene, as fe_set_wall / fe_get_wall: the scene's box comes from its
  * wall at the start of every frame (the per-scene constants are copied to the device on the
  * context's stream, also before graph replays). */
