@@ -392,6 +392,7 @@ def divergence_order_measurement(fe, w, opts, steps, warmup, tuned=None):
         d, v, frames = sim.iteration_totals()
         sim.close()
         out[name] = {"steps_per_s": round(steps / (ms / 1e3), 3),
+                     "tuned_blocks": {k: v["block"] for k, v in tuned.items()},
                      "sweeps_per_step": {"density": round(d / max(frames, 1), 3),
                                          "divergence": round(v / max(frames, 1), 3)}}
     return out
@@ -451,7 +452,8 @@ def app_frame_measurement(fe, scene, with_cpu, warmup=20, frames=200):
 
 def boundary_measurement(fe, w, opts, steps, warmup):
     """The same frames with the reference's clamp-only walls and with wall boundary particles
-    (FE_BOUNDARY_PARTICLES, a non-parity mode): steps/s, Jacobi sweeps per step, and after the
+    (FE_BOUNDARY_PARTICLES, a non-parity mode), each autotuned after the warm-up: steps/s,
+    Jacobi sweeps per step, and after the
     frames the mean density of the fluid layer on the floor (particles within 2 r of it) and the
     number of particles the floor's collision clamp holds."""
     import numpy as np
@@ -463,6 +465,9 @@ def boundary_measurement(fe, w, opts, steps, warmup):
         o.boundary = b
         sim = fe.Simulation(w.params, w.positions, device=0, options=o)
         sim.run(warmup, w.params.dt)
+        # each mode tuned on its own state (with the samples on, the tuner times the walled
+        # gather instances)
+        tuned = sim.autotune(None, reps=10)
         sim.reset_iteration_totals()
         sim.timer_start()
         sim.run(steps, w.params.dt)
@@ -476,6 +481,7 @@ def boundary_measurement(fe, w, opts, steps, warmup):
         # resting on the collision clamp: the clamp puts a particle at box_min.y + r + 0.001
         on_clamp = pos_a[:, 1] <= float(w.params.box_min[1]) + r + 0.0015
         out[name] = {"steps_per_s": round(steps / (ms / 1e3), 3),
+                     "tuned_blocks": {k: v["block"] for k, v in tuned.items()},
                      "sweeps_per_step": {"density": round(d / max(frames, 1), 3),
                                          "divergence": round(v / max(frames, 1), 3)},
                      "floor_layer": {"particles": int(floor.sum()),

@@ -1614,10 +1614,11 @@ hipError_t ensure_wall_g(fe_ctx* c) {
     return hipSuccess;
 }
 
-// A gather with the wall samples: one configuration (256 threads, one scene, the
-// per-particle lists or the bucket walk).
+// A gather with the wall samples (one scene, the per-particle lists or the bucket walk), in
+// the block size tuned for kernel `tid` (the tuner times these instances when the walls are
+// on); tid < 0 (the feature gathers) runs 256 threads.
 template <class P>
-hipError_t run_walled(fe_ctx* c, const char* name, const P& pol, const unsigned long long* prev = nullptr,
+hipError_t run_walled(fe_ctx* c, const char* name, int tid, const P& pol, const unsigned long long* prev = nullptr,
                       unsigned long long thr = 0, uint32_t kind = 0, unsigned long long* slot_out = nullptr) {
     hipError_t e = ensure_walls(c);
     if (e == hipSuccess) e = ensure_wall_g(c);
@@ -1626,13 +1627,16 @@ hipError_t run_walled(fe_ctx* c, const char* name, const P& pol, const unsigned 
     static_cast<P&>(wp) = pol;
     wp.wall_g = c->wall_g;
     Launch L(c, name);
-    launch_gather<Walled<P>, false, 256>(c, wp, prev, thr, kind, slot_out);
+    if (tid < 0)
+        launch_gather<Walled<P>, false, 256>(c, wp, prev, thr, kind, slot_out);
+    else
+        launch_gather_b<Walled<P>, false>(c, c->tune[tid].block, wp, prev, thr, kind, slot_out);
     return hipGetLastError();
 }
 
 hipError_t launch_density_alpha_only(fe_ctx* c) {
     DensityAlpha pol{c->dens, c->fac, c->dens_m, c->fac_m, c->vb_m};
-    hipError_t e = walls_on(c) ? run_walled(c, "density_alpha", pol) : run_list(c, "density_alpha", kTuneDensityAlpha, pol);
+    hipError_t e = walls_on(c) ? run_walled(c, "density_alpha", kTuneDensityAlpha, pol) : run_list(c, "density_alpha", kTuneDensityAlpha, pol);
     c->mirrors = (c->mirrors | kMirrorDens | kMirrorFac | kMirrorMrho) & ~kMirrorPrho;   // p / rho^2 used the old densities
     c->da_fresh = c->ns_fresh;
     return e;
@@ -1674,11 +1678,11 @@ hipError_t launch_density_alpha_divergence_source(fe_ctx* c) {
     if (learn_features_wanted(c, 1)) {
         if ((e = learn_prev(c, 1)) != hipSuccess) return e;
         DensityAlphaDivergenceFeat fp{pol.da, pol.dv, c->ws.pprev, c->ws.feat, c->n};
-        e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source_feat", fp)
+        e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source_feat", -1, fp)
                         : run_list(c, "density_alpha_divergence_source_feat", kTuneDensityAlphaDivergence, fp);
         learn_features_done(c, 1);
     } else {
-        e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source", pol)
+        e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source", kTuneDensityAlphaDivergence, pol)
                         : run_list(c, "density_alpha_divergence_source", kTuneDensityAlphaDivergence, pol);
     }
     density_alpha_divergence_done(c);
@@ -1689,7 +1693,7 @@ hipError_t launch_density_alpha_divergence_only(fe_ctx* c) {
     hipError_t e = density_alpha_divergence_mirrors(c);
     if (e != hipSuccess) return e;
     const DensityAlphaDivergence pol = density_alpha_divergence_of(c);
-    e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source", pol)
+    e = walls_on(c) ? run_walled(c, "density_alpha_divergence_source", kTuneDensityAlphaDivergence, pol)
                     : run_list(c, "density_alpha_divergence_source", kTuneDensityAlphaDivergence, pol);
     density_alpha_divergence_done(c);
     return e;
@@ -1711,11 +1715,11 @@ hipError_t launch_density_source(fe_ctx* c) {
     if (learn_features_wanted(c, 0)) {
         if ((e = learn_prev(c, 0)) != hipSuccess) return e;
         DensitySourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n};
-        e = walls_on(c) ? run_walled(c, "density_source_feat", fp)
+        e = walls_on(c) ? run_walled(c, "density_source_feat", -1, fp)
                         : run_list(c, "density_source_feat", kTuneDensitySource, fp);
         learn_features_done(c, 0);
     } else {
-        e = walls_on(c) ? run_walled(c, "density_source", pol) : run_list(c, "density_source", kTuneDensitySource, pol);
+        e = walls_on(c) ? run_walled(c, "density_source", kTuneDensitySource, pol) : run_list(c, "density_source", kTuneDensitySource, pol);
     }
     c->mirrors |= kMirrorPrho | kMirrorSrc | kMirrorPres;
     return e;
@@ -1731,11 +1735,11 @@ hipError_t launch_divergence_source(fe_ctx* c) {
     if (learn_features_wanted(c, 1)) {
         if ((e = learn_prev(c, 1)) != hipSuccess) return e;
         DivergenceSourceFeat fp{pol, c->fac, c->ws.pprev, c->ws.feat, c->n};
-        e = walls_on(c) ? run_walled(c, "divergence_source_feat", fp)
+        e = walls_on(c) ? run_walled(c, "divergence_source_feat", -1, fp)
                         : run_list(c, "divergence_source_feat", kTuneDivergenceSource, fp);
         learn_features_done(c, 1);
     } else {
-        e = walls_on(c) ? run_walled(c, "divergence_source", pol)
+        e = walls_on(c) ? run_walled(c, "divergence_source", kTuneDivergenceSource, pol)
                         : run_list(c, "divergence_source", kTuneDivergenceSource, pol);
     }
     c->mirrors |= kMirrorPrho | kMirrorSrc | kMirrorPres;
@@ -1746,7 +1750,7 @@ hipError_t launch_pressure_force(fe_ctx* c, const unsigned long long* prev, unsi
     hipError_t e = need(c, kMirrorPos | kMirrorPrho);
     if (e != hipSuccess) return e;
     PressureForce pol{c->defer_sweep_writes ? nullptr : c->pacc, c->pa_m, c->pa2_m};
-    e = walls_on(c) ? run_walled(c, "pressure_force", pol, prev, thr, kind, nullptr)
+    e = walls_on(c) ? run_walled(c, "pressure_force", kTunePressureForce, pol, prev, thr, kind, nullptr)
                     : run_sweep(c, "pressure_force", kTunePressureForce, pol, prev, thr, kind, nullptr);
     c->mirrors |= kMirrorPacc;
     c->pending_pacc |= c->defer_sweep_writes;
@@ -1759,7 +1763,7 @@ hipError_t launch_pressure_update(fe_ctx* c, unsigned long long* out, const unsi
     if (e != hipSuccess) return e;
     PressureUpdate pol{c->pa_m,   c->pa2_m,  c->fac_m, c->src_m, c->dens_m, c->defer_sweep_writes ? nullptr : c->pres,
                        c->pres_m, c->pos_m, c->options.residual == FE_RESIDUAL_TWO_SIDED ? 1u : 0u};
-    e = walls_on(c) ? run_walled(c, "pressure_update", pol, prev, thr, kind, out)
+    e = walls_on(c) ? run_walled(c, "pressure_update", kTunePressureUpdate, pol, prev, thr, kind, out)
                     : run_sweep(c, "pressure_update", kTunePressureUpdate, pol, prev, thr, kind, out);
     c->pending_pres |= c->defer_sweep_writes;
     return e;
@@ -1772,7 +1776,7 @@ hipError_t launch_residual_report(fe_ctx* c, int kind) {
     static_cast<PressureUpdate&>(pol) = PressureUpdate{c->pa_m, c->pa2_m, c->fac_m, c->src_m, c->dens_m, nullptr,
                                                        c->pres_m, c->pos_m, 0u};
     pol.err_out = c->report_err;
-    e = walls_on(c) ? run_walled(c, "residual_report", pol) : run_list(c, "residual_report", kTunePressureUpdate, pol);
+    e = walls_on(c) ? run_walled(c, "residual_report", -1, pol) : run_list(c, "residual_report", kTunePressureUpdate, pol);
     if (e != hipSuccess) return e;
     Launch L(c, "residual_reduce");
     hipLaunchKernelGGL(k_residual_reduce, grid_for(c->n), dim3(kBlock), 0, c->stream, c->report_err, c->pres_m, c->n,

@@ -102,3 +102,30 @@ def test_autotune_batch():
     finally:
         ref.close()
         tun.close()
+
+
+@pytest.mark.parametrize("block", [64, 128, 512, 1024])
+def test_walled_gathers_take_tuned_blocks(block):
+    """With wall boundary particles the gathers run the walled instances in the block size
+    tuned for each kernel: every block size gives the default configuration's frames, and
+    tuning the walled context keeps its next frames."""
+    w = scene.dam_break_32k(seed=8)
+    o = conv()
+    o.boundary = fe.FE_BOUNDARY_PARTICLES
+    names = ("density_alpha", "density_source", "divergence_source", "pressure_force", "pressure_update",
+             "density_alpha_divergence_source")
+    with fe.Simulation(w.params, w.positions, device=0, options=o) as ref, \
+            fe.Simulation(w.params, w.positions, device=0, options=o) as alt:
+        for k in names:
+            alt.set_tune(k, block, 1)
+        ref.run(3)
+        alt.run(3)
+        for name in FIELDS:
+            assert same(alt.download(name), ref.download(name)), name
+        assert alt.iteration_totals() == ref.iteration_totals()
+        if block == 64:
+            alt.autotune(None, reps=3)
+            ref.run(2)
+            alt.run(2)
+            for name in FIELDS:
+                assert same(alt.download(name), ref.download(name)), name
