@@ -649,6 +649,23 @@ def learned_comparison(fe, w, opts, collect, evaluate, tuned=None, report_frames
         res["sweep_reduction"] = round((rs["density"] + rs["divergence"]) / (ls["density"] + ls["divergence"]), 3)
         res["speedup"] = round(res["learned"]["steps_per_s"] / res["reference"]["steps_per_s"], 3)
         modes[mode] = res
+    # the density network alone: the divergence solve keeps the reference start, which
+    # already needs few sweeps, and skips that network's inference and feature gathers
+    sim = make(False)
+    o = fe.FeOptions.from_buffer_copy(opts)
+    o.residual = fe.FE_RESIDUAL_TWO_SIDED
+    sim.learn_set_weights(fe.FE_NET_DENSITY, *nets[fe.FE_NET_DENSITY])
+    o.warm_start = fe.FE_WARM_START_LEARNED
+    sim.set_options(o)
+    sim.reset_iteration_totals()
+    sim.timer_start()
+    sim.run(evaluate)
+    ms = sim.timer_stop()
+    d, v, f = sim.iteration_totals()
+    sim.close()
+    density_only = {"steps_per_s": round(evaluate / (ms / 1e3), 3),
+                    "sweeps_per_step": {"density": round(d / f, 3), "divergence": round(v / f, 3)}}
+    density_only["speedup"] = round(density_only["steps_per_s"] / modes["two_sided"]["reference"]["steps_per_s"], 3)
     out = {
         "workload": f"{w.name}: frames {collect}-{collect + evaluate - 1} after sampling frames 0-{collect - 1}",
         "samples_per_network": collect * 2048,
@@ -664,6 +681,7 @@ def learned_comparison(fe, w, opts, collect, evaluate, tuned=None, report_frames
         "sweep_reduction": modes["two_sided"]["sweep_reduction"],
         "speedup": modes["two_sided"]["speedup"],
         "compression_residual": modes["compression"],
+        "learned_density_only": density_only,
     }
     if "ws_infer" in prof:
         launches, total_ms = prof["ws_infer"]
