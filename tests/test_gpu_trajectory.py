@@ -1,7 +1,7 @@
 """Long trajectories on the GPU, through the C ABI: the 32k dam break (32 x 32 x 32 block,
-spacing 0.02 m, in a 1 x 1 x 1 m box, dt 1 ms) for 200 frames from rest (FE_TRAJECTORY_FRAMES
-sets another count, e.g. the workload's full 1000 steps) equals the oracle's trajectory bit
-for bit. The stats and sweep counts match at every frame and every field
+spacing 0.02 m, in a 1 x 1 x 1 m box, dt 1 ms) for the workload's full 1000 steps from rest
+(FE_TRAJECTORY_FRAMES sets another count) equals the oracle's trajectory bit for bit; both runs
+take about 80 s on one MI355X. The stats and sweep counts match at every frame and every field
 matches every 25 frames. The runs cover converge mode with the radix sort and the reference's
 fixed 4/4 sweeps with the bitonic sort. Rounding differences compound over frames, so one
 differing bit anywhere on the way would show at the next checkpoint.
@@ -16,7 +16,7 @@ from fluid_engine import scene
 
 pytestmark = pytest.mark.gpu
 
-FRAMES = int(os.environ.get("FE_TRAJECTORY_FRAMES", "200"))
+FRAMES = int(os.environ.get("FE_TRAJECTORY_FRAMES", "1000"))
 FIELDS = ("position_a", "velocity_a", "velocity_b", "position_b", "densities", "factors", "source_terms",
           "pressures", "pressure_accelerations", "colors", "grid_entries", "grid_start")
 
