@@ -364,8 +364,10 @@ fe_status jacobi(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t*
     for (uint32_t k = 0; S > 1 && k < S; k++)
         thr_s[k] = (unsigned long long)((double)tol * (double)c->scene_n[k] * 4294967296.0);
     FE_HIP(fe::launch_fill_u64(c, slot, S > 1 ? S * fe::kMaxSolveIters : maxit), "fill residuals");
-    // Chunk sizes: the previous frame's sweep count first (the usual answer), then 2, 4, 8,
-    // 16, ... Chunk j's residuals are read back while chunk j+1 runs.
+    // Chunk sizes: the previous frame's sweep count first (the usual answer), then 1, 2, 4,
+    // 8, 16, ... Chunk j's residuals are read back while chunk j+1 runs, so a solve that ends
+    // with chunk j also launches chunk j+1, whose sweeps return at once (about 4.5 us a
+    // kernel at 1M): a one-sweep second chunk keeps that to two kernels.
     struct Chunk {
         hipEvent_t ev;
         uint32_t beg, end;
@@ -386,7 +388,7 @@ fe_status jacobi(fe_ctx* c, int kind, uint32_t fixed_iters, float tol, uint32_t*
     };
     uint32_t first = c->last_iters[kind] ? c->last_iters[kind] : 4;
     if (first > maxit) first = maxit;
-    uint32_t chunk_beg = 0, chunk_end = first, grow = 2;
+    uint32_t chunk_beg = 0, chunk_end = first, grow = 1;
     uint32_t launched = 0;
     size_t checked_chunks = 0;
     fe_status st = FE_OK;

@@ -149,11 +149,9 @@ __global__ __launch_bounds__(kBlock) void k_refresh_mirrors(Sph s0, Scenes sc, c
 
 // ---- neighbour list build -----------------------------------------------------------------
 
-// Entry `slot` of work-item t in the tiled list: tiles of 16 work-items x 4 slots (256 B),
-// so 16 lanes reading 4 slots each touch 256 contiguous bytes.
-__device__ __forceinline__ size_t list_at(uint32_t t, uint32_t slot) {
-    return ((((size_t)(t >> 4) * (kMaxNbr / 4) + (slot >> 2)) << 6) + ((t & 15u) << 2) + (slot & 3u));
-}
+// The tiled list: entry `slot` of work-item t sits at
+// ((t / 16) (kMaxNbr / 4) + slot / 4) 64 + (t % 16) 4 + slot % 4, in tiles of 16 work-items x 4
+// slots (256 B), so 16 lanes reading 4 slots each touch 256 contiguous bytes.
 
 constexpr uint32_t kCand = 256;   // candidates staged per pass per wave
 
@@ -203,12 +201,15 @@ __device__ void build_group(const Sph& s, const uint2* __restrict__ range, const
     const uint64_t segmask = S == 64 ? ~0ull : ((1ull << S) - 1ull);
     const uint32_t sub = (uint32_t)lane & (S - 1u);
     const uint32_t seg_shift = (uint32_t)lane & ~(S - 1u);
+    const uint64_t segabs = segmask << seg_shift;   // this lane's particle's lanes
     uint32_t staged = 0xFFFFFFFFu;
 
     for (uint32_t pb = gs; pb < ge; pb += per_chunk) {
         const uint32_t tp = pb + (uint32_t)lane / S;
         const bool act = tp < ge;
         const float4 pi = pos_m[act ? tp : gs];
+        // tp's row of the tiled list: slot sl is at ((sl / 4) 64 + sl % 4) from it
+        uint32_t* const row = nbr + ((((size_t)(tp >> 4) * (kMaxNbr / 4)) << 6) + ((tp & 15u) << 2));
         uint32_t m = 0;
         for (uint32_t cb = 0; cb < C; cb += kCand) {
             const uint32_t cn = (C - cb) < kCand ? (C - cb) : kCand;
@@ -236,10 +237,12 @@ __device__ void build_group(const Sph& s, const uint2* __restrict__ range, const
                     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
                     ok = !(rx * rx + ry * ry + rz * rz > s.hh);
                 }
-                const uint64_t seg = (__ballot(ok) >> seg_shift) & segmask;
+                // the particle's accepted candidates, numbered in lane order
+                const uint64_t seg = __ballot(ok) & segabs;
                 if (ok) {
-                    const uint32_t slot = m + (uint32_t)__popcll(seg & ((1ull << sub) - 1ull));
-                    if (slot < kMaxNbr) nbr[list_at(tp, slot)] = st.idx[q];
+                    const uint32_t slot = m + __builtin_amdgcn_mbcnt_hi((uint32_t)(seg >> 32),
+                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)seg, 0u));
+                    if (slot < kMaxNbr) row[((slot >> 2) << 6) + (slot & 3u)] = st.idx[q];
                 }
                 m += (uint32_t)__popcll(seg);
             }
