@@ -184,32 +184,36 @@ static_assert(PK_END <= kMlpPackedFloats, "packed layout");
 
 // GELU of the inference kernel: piecewise linear through GELU at z = -8 + k / 128,
 // k = 0 .. 2048, with the end segments extended linearly (GELU(z) is 0 below -8 and z above 8
-// to within 1e-14). In the table coordinate u = (z + 8) 128, entry k holds segment k as
-// (intercept, slope): GELU ~ intercept + slope u on [k, k + 1].
+// to within 1e-14). Inference works in the coordinate U = 1024 z + 8192 = 8 u, u = (z + 8) 128
+// the segment coordinate: the packed weights are scaled by 1024 (exact in BF16) and the packed
+// biases are 1024 b + 8192, and the bias enters as the products' initial accumulator, so each
+// layer's accumulators are U. Entry k holds segment k as (intercept, slope / 8):
+// GELU ~ intercept + (slope / 8) U on u in [k, k + 1], and its byte offset in the table is
+// floor(U) rounded down to a multiple of 8.
 // Interpolation error <= (1/128)^2 / 8 * max|GELU''| < 7e-6, far below a BF16 step of the
 // activations it feeds; the table is built from gelu() above.
 constexpr int kGeluNodes = 2048;
 constexpr float kGeluLo = -8.0f, kGeluScale = 128.0f;
+constexpr float kUScale = 8.0f * kGeluScale, kUBias = -8.0f * kGeluLo * kGeluScale;   // U = 1024 z + 8192
 
 __global__ __launch_bounds__(256) void k_gelu_table(float2* __restrict__ table) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < kGeluNodes; k += gridDim.x * blockDim.x) {
         const double z0 = (double)kGeluLo + (double)k / kGeluScale, z1 = z0 + 1.0 / kGeluScale;
         const float g0 = gelu((float)z0), g1 = gelu((float)z1);
         const float rise = g1 - g0;
-        table[k] = make_float2((float)((double)g0 - (double)rise * k), rise);
+        table[k] = make_float2((float)((double)g0 - (double)rise * k), (float)((double)rise / 8.0));
     }
 }
 
 static_assert(PK_END + 2 * kGeluNodes <= kMlpPackedFloats, "packed layout with the GELU table");
 
-__device__ __forceinline__ float gelu_lut(const float2* __restrict__ lut, float z) {
-    const float u = fmaf(z, kGeluScale, -kGeluLo * kGeluScale);
-    // the segment of u; truncation equals floor once negative u clamp to segment 0
-    // (v_cvt_i32_f32: NaN -> 0, saturating)
-    int k = (int)u;
-    k = k < 0 ? 0 : (k > kGeluNodes - 1 ? kGeluNodes - 1 : k);
-    const float2 e = lut[k];
-    return fmaf(e.y, u, e.x);
+// GELU at the table coordinate U (see above): the segment's byte offset is the truncation of
+// U clamped to the table and rounded down to 8 bytes (v_cvt_i32_f32: NaN -> 0, saturating).
+__device__ __forceinline__ float gelu_lut(const float2* __restrict__ lut, float U) {
+    int b = (int)U;
+    b = b < 0 ? 0 : (b > 8 * kGeluNodes - 1 ? 8 * kGeluNodes - 1 : b);
+    const float2 e = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(lut) + (b & ~7));
+    return fmaf(e.y, U, e.x);
 }
 
 // Operand g: layer 1 tiles g = 0..3 (column tile ct = g), layer 2 g = 4 + 2 ct + kc, layer 3
@@ -231,12 +235,12 @@ __global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ para
         float v = 0.0f;
         if (layer == 0) v = in < kMlpIn ? params[Off::W1 + in * H + out] : 0.0f;
         else v = params[(layer == 1 ? Off::W2 : Off::W3) + in * H + out];
-        ops[e] = tobf(v);
+        ops[e] = tobf(v * kUScale);   // = 1024 bf16(v): a power of two commutes with the rounding
     }
     for (int e = threadIdx.x; e < H; e += blockDim.x) {
-        packed[PK_B1 + e] = params[Off::B1 + e];
-        packed[PK_B2 + e] = params[Off::B2 + e];
-        packed[PK_B3 + e] = params[Off::B3 + e];
+        packed[PK_B1 + e] = fmaf(params[Off::B1 + e], kUScale, kUBias);
+        packed[PK_B2 + e] = fmaf(params[Off::B2 + e], kUScale, kUBias);
+        packed[PK_B3 + e] = fmaf(params[Off::B3 + e], kUScale, kUBias);
         packed[PK_W4 + e] = (float)tobf(params[Off::W4 + e]);
     }
     for (int e = threadIdx.x; e < kMlpIn; e += blockDim.x) {
@@ -264,15 +268,19 @@ __device__ __forceinline__ void load_inputs(const float* __restrict__ feat, uint
     }
 }
 
-// One layer's bias + GELU on the accumulators of the four column tiles, written as BF16
+// The accumulators' initial values: this lane's four neurons' packed biases (U coordinate).
+__device__ __forceinline__ f32x4 bias4(const float* sb, int ct, int q) {
+    const float4 b = *reinterpret_cast<const float4*>(sb + ct * 16 + 4 * q);
+    return f32x4{b.x, b.y, b.z, b.w};
+}
+
+// One layer's GELU on the accumulators (U) of the four column tiles, written as BF16
 // activation rows act[sample][neuron].
-__device__ __forceinline__ void store_act(const float2* lut, const float* sb, const f32x4 (&acc)[4], __bf16* act,
-                                          int c, int q) {
+__device__ __forceinline__ void store_act(const float2* lut, const f32x4 (&acc)[4], __bf16* act, int c, int q) {
 #pragma unroll
     for (int ct = 0; ct < 4; ct++) {
-        const float4 b = *reinterpret_cast<const float4*>(sb + ct * 16 + 4 * q);
-        const __bf16 h0 = tobf(gelu_lut(lut, acc[ct][0] + b.x)), h1 = tobf(gelu_lut(lut, acc[ct][1] + b.y));
-        const __bf16 h2 = tobf(gelu_lut(lut, acc[ct][2] + b.z)), h3 = tobf(gelu_lut(lut, acc[ct][3] + b.w));
+        const __bf16 h0 = tobf(gelu_lut(lut, acc[ct][0])), h1 = tobf(gelu_lut(lut, acc[ct][1]));
+        const __bf16 h2 = tobf(gelu_lut(lut, acc[ct][2])), h3 = tobf(gelu_lut(lut, acc[ct][3]));
         typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
         const bf16x4 v = {h0, h1, h2, h3};
         *reinterpret_cast<bf16x4*>(act + c * H + ct * 16 + 4 * q) = v;
@@ -329,8 +337,8 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const float* __r
         for (int e = 0; e < 8; e++) B[e] = tobf((x[e] - mu[e]) * is[e]);
         f32x4 acc[4];
 #pragma unroll
-        for (int ct = 0; ct < 4; ct++) acc[ct] = mfma(A1[ct], B, f32x4{0.f, 0.f, 0.f, 0.f});
-        store_act(lut, sf + (PK_B1 - PK_B1), acc, a, c, q);
+        for (int ct = 0; ct < 4; ct++) acc[ct] = mfma(A1[ct], B, bias4(sf + (PK_B1 - PK_B1), ct, q));
+        store_act(lut, acc, a, c, q);
         wave_lds_fence();
 #pragma unroll
         for (int layer = 0; layer < 2; layer++) {
@@ -341,10 +349,10 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const float* __r
             for (int ct = 0; ct < 4; ct++) {
                 const bf16x8 a0 = as_bf16x8(sops[(8 * layer + 2 * ct) * 64 + l]);
                 const bf16x8 a1 = as_bf16x8(sops[(8 * layer + 2 * ct + 1) * 64 + l]);
-                acc[ct] = mfma(a1, B1, mfma(a0, B0, f32x4{0.f, 0.f, 0.f, 0.f}));
+                acc[ct] = mfma(a1, B1, mfma(a0, B0, bias4(sf + (layer == 0 ? PK_B2 : PK_B3) - PK_B1, ct, q)));
             }
             if (layer == 0) {
-                store_act(lut, sf + (PK_B2 - PK_B1), acc, a, c, q);
+                store_act(lut, acc, a, c, q);
                 wave_lds_fence();
             }
         }
@@ -353,12 +361,11 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const float* __r
         float y = 0.0f;
 #pragma unroll
         for (int ct = 0; ct < 4; ct++) {
-            const float4 b = *reinterpret_cast<const float4*>(sf + (PK_B3 - PK_B1) + ct * 16 + 4 * q);
             const float4 w4 = *reinterpret_cast<const float4*>(sf + (PK_W4 - PK_B1) + ct * 16 + 4 * q);
-            y += (float)tobf(gelu_lut(lut, acc[ct][0] + b.x)) * w4.x;
-            y += (float)tobf(gelu_lut(lut, acc[ct][1] + b.y)) * w4.y;
-            y += (float)tobf(gelu_lut(lut, acc[ct][2] + b.z)) * w4.z;
-            y += (float)tobf(gelu_lut(lut, acc[ct][3] + b.w)) * w4.w;
+            y += (float)tobf(gelu_lut(lut, acc[ct][0])) * w4.x;
+            y += (float)tobf(gelu_lut(lut, acc[ct][1])) * w4.y;
+            y += (float)tobf(gelu_lut(lut, acc[ct][2])) * w4.z;
+            y += (float)tobf(gelu_lut(lut, acc[ct][3])) * w4.w;
         }
         y += __shfl_xor(y, 16, 64);
         y += __shfl_xor(y, 32, 64);
