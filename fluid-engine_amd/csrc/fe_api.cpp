@@ -475,8 +475,8 @@ fe_status neighbor_search(fe_ctx* c) {
 
 // The frame's stats (and, with the watchdog, its health words) read back behind the frame.
 fe_status enqueue_stats_copy(fe_ctx* c) {
-    FE_HIP(hipMemcpyAsync(c->host_stats, c->stats, c->watchdog ? 24 : 12, hipMemcpyDeviceToHost, c->stream),
-           "stats copy");
+    // words 0-2 the stats, 4-5 the watchdog's, 6 the radix sort's error word
+    FE_HIP(hipMemcpyAsync(c->host_stats, c->stats, 28, hipMemcpyDeviceToHost, c->stream), "stats copy");
     FE_HIP(hipEventRecord(c->stats_event, c->stream), "stats event");
     c->stats_pending = true;
     c->stats_frame = c->frames;
@@ -489,6 +489,7 @@ fe_status wait_prev_stats(fe_ctx* c) {
         FE_HIP(hipEventSynchronize(c->stats_event), "stats wait");
         std::memcpy(c->prev_stats, c->host_stats, sizeof(c->prev_stats));
         c->stats_pending = false;
+        if (c->host_stats[6]) c->sort_failed = true;
         if (c->stats_health) {
             fe_health& h = c->health;
             h.frame = c->stats_frame;
@@ -498,6 +499,7 @@ fe_status wait_prev_stats(fe_ctx* c) {
             h.speed_limit = c->speed_limit;
             h.flags = (h.nonfinite ? FE_HEALTH_NONFINITE : 0u) |
                       (c->speed_limit > 0.0f && !(h.max_speed <= c->speed_limit) ? FE_HEALTH_SPEED : 0u);
+            h.sort_error = c->host_stats[6];
         }
     }
     return FE_OK;
@@ -507,19 +509,6 @@ fe_status wait_prev_stats(fe_ctx* c) {
 // its replays).
 fe_status step(fe_ctx* c, float max_dt, bool stats_copy = true) {
     uint32_t dit = c->params.density_solver_iterations, vit = c->params.divergence_solver_iterations;
-    if (c->watchdog && c->watchdog_halt) {
-        fe_status s0 = wait_prev_stats(c);
-        if (s0 != FE_OK) return s0;
-        const fe_health& h = c->health;
-        if (h.flags & FE_HEALTH_NONFINITE)
-            return fail(FE_ERR_UNSTABLE, "watchdog: frame " + std::to_string(h.frame) + " left " +
-                                             std::to_string(h.nonfinite) + " particles with a non-finite field (first: " +
-                                             std::to_string(h.first_nonfinite) + ")");
-        if (h.flags & FE_HEALTH_SPEED)
-            return fail(FE_ERR_UNSTABLE, "watchdog: frame " + std::to_string(h.frame) + " max speed " +
-                                             std::to_string(h.max_speed) + " exceeds the limit " +
-                                             std::to_string(h.speed_limit));
-    }
     if (c->options.use_cfl || c->options.use_solver_error_threshold) {
         fe_status s = wait_prev_stats(c);
         if (s != FE_OK) return s;
@@ -604,6 +593,40 @@ fe_status step_graph(fe_ctx* c, float max_dt) {
     return FE_OK;
 }
 
+// Before a frame (and outside any capture): a finished frame's stats are taken without
+// waiting; a radix sort whose look-back stopped waiting left a wrong permutation, so the
+// context fails from then on; with the watchdog's halt, an unhealthy previous frame stops this
+// one.
+fe_status frame_checks(fe_ctx* c) {
+    if (c->stats_pending) {
+        const hipError_t q = hipEventQuery(c->stats_event);
+        if (q == hipSuccess) {
+            fe_status s0 = wait_prev_stats(c);
+            if (s0 != FE_OK) return s0;
+        } else {
+            (void)hipGetLastError();   // "not ready" must not reach the launch checks that follow
+            if (q != hipErrorNotReady) return hip_fail(q, "stats query");
+        }
+    }
+    if (c->sort_failed)
+        return fail(FE_ERR_DEVICE, "a radix sort's look-back stopped waiting for an earlier tile; the neighbour "
+                                   "search is not trustworthy (recreate the context)");
+    if (c->watchdog && c->watchdog_halt) {
+        fe_status s0 = wait_prev_stats(c);
+        if (s0 != FE_OK) return s0;
+        const fe_health& h = c->health;
+        if (h.flags & FE_HEALTH_NONFINITE)
+            return fail(FE_ERR_UNSTABLE, "watchdog: frame " + std::to_string(h.frame) + " left " +
+                                             std::to_string(h.nonfinite) + " particles with a non-finite field (first: " +
+                                             std::to_string(h.first_nonfinite) + ")");
+        if (h.flags & FE_HEALTH_SPEED)
+            return fail(FE_ERR_UNSTABLE, "watchdog: frame " + std::to_string(h.frame) + " max speed " +
+                                             std::to_string(h.max_speed) + " exceeds the limit " +
+                                             std::to_string(h.speed_limit));
+    }
+    return FE_OK;
+}
+
 // Whether a single scene's frame can be replayed from a graph: fixed sweeps and nothing that
 // makes the host decide inside the frame (frame control, learning, exit reports) or changes
 // kernel arguments from frame to frame other than the box (wall samples follow the box).
@@ -615,8 +638,9 @@ bool single_graph_ok(const fe_ctx* c) {
 }
 
 fe_status run_frame(fe_ctx* c, float max_dt) {
-    fe_status s = apply_walls(c);
+    fe_status s = frame_checks(c);
     if (s != FE_OK) return s;
+    if ((s = apply_walls(c)) != FE_OK) return s;
     const bool graph = c->batch_graph && c->options.solver_mode == FE_SOLVER_FIXED && !c->profiling &&
                        (c->nscenes > 1 || single_graph_ok(c));
     if (graph && c->nscenes == 1) {

@@ -288,6 +288,7 @@ struct fe_ctx {
     uint32_t prev_stats[3] = {0, 0, 0};
     uint64_t stats_frame = 0;      // frames run when the pending stats were enqueued
     bool stats_health = false;     // the pending copy holds the health words
+    bool sort_failed = false;      // a radix sort's look-back gave up (stats word 6)
     // the watchdog (fe_set_watchdog): health words in stats[4..5], the last report
     bool watchdog = false, watchdog_halt = false;
     float speed_limit = 0.0f;

@@ -347,9 +347,12 @@ __global__ __launch_bounds__(kBlock) void k_stats(uint32_t n, const float4* __re
     }
 }
 
+// Word 6 of the stats buffer carries the radix sort's error word (a look-back that stopped
+// waiting), so the host sees it with the frame's stats.
 template <bool kHealth>
 __global__ __launch_bounds__(kBlock) void k_stats_final(const uint32_t* __restrict__ partial, uint32_t blocks,
-                                                        uint32_t* __restrict__ stats) {
+                                                        uint32_t* __restrict__ stats,
+                                                        const uint32_t* __restrict__ sort_err) {
     __shared__ uint32_t part[5][kBlock / 64];
     uint32_t mx = 0, s1 = 0, s2 = 0, bad = 0, first = 0xFFFFFFFFu;
     for (uint32_t k = threadIdx.x; k < blocks; k += kBlock) {
@@ -392,6 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_stats_final(const uint32_t* __restri
             stats[4] = bad;
             stats[5] = first;
         }
+        stats[6] = *sort_err;
         stats[0] = mx;
         stats[1] = s1;
         stats[2] = s2;
@@ -729,11 +733,13 @@ hipError_t launch_stats(fe_ctx* c) {
                                c->src, c->params.target_density, c->stats_partial, c->pos_a, c->pres);
     }
     Launch L(c, "stats_final");
+    const uint32_t* sort_err = c->sort.ctrl + kSortMaxPasses;
     if (c->watchdog)
-        hipLaunchKernelGGL(k_stats_final<true>, dim3(1), dim3(kBlock), 0, c->stream, c->stats_partial, blocks, c->stats);
+        hipLaunchKernelGGL(k_stats_final<true>, dim3(1), dim3(kBlock), 0, c->stream, c->stats_partial, blocks, c->stats,
+                           sort_err);
     else
         hipLaunchKernelGGL(k_stats_final<false>, dim3(1), dim3(kBlock), 0, c->stream, c->stats_partial, blocks,
-                           c->stats);
+                           c->stats, sort_err);
     return hipGetLastError();
 }
 

@@ -235,7 +235,7 @@ class Simulation:
         h = FeHealth()
         check(self._lib.fe_get_health(self._ctx, byref(h)), "fe_get_health")
         return {"frame": h.frame, "nonfinite": h.nonfinite, "first_nonfinite": h.first_nonfinite,
-                "max_speed": h.max_speed, "speed_limit": h.speed_limit, "flags": h.flags}
+                "max_speed": h.max_speed, "speed_limit": h.speed_limit, "flags": h.flags, "sort_error": h.sort_error}
 
     def use_graph(self, enable: bool = True):
         """Replay the steady-state FIXED frame from a captured HIP graph (fe_use_graph)."""

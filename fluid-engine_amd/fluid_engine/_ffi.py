@@ -128,7 +128,7 @@ class FeHealth(Structure):
         ("max_speed", c_float),
         ("speed_limit", c_float),
         ("flags", c_uint32),
-        ("reserved", c_uint32),
+        ("sort_error", c_uint32),
     ]
 
 

@@ -420,7 +420,8 @@ typedef struct fe_health {
     float max_speed;           /* the frame's max |velocity_a| (the stats buffer's word 0) */
     float speed_limit;
     uint32_t flags;            /* FE_HEALTH_NONFINITE | FE_HEALTH_SPEED */
-    uint32_t reserved;
+    uint32_t sort_error;       /* nonzero: a radix sort's look-back stopped waiting (any context
+                                  with it fails its next fe_step with FE_ERR_DEVICE) */
 } fe_health;
 fe_status fe_set_watchdog(fe_ctx* ctx, int enable, float speed_limit, int halt);
 fe_status fe_get_health(fe_ctx* ctx, fe_health* out);

@@ -35,6 +35,7 @@ def test_healthy_frames_report_nothing_and_change_nothing():
             h = b.health()
             assert h["frame"] == f + 1
             assert h["nonfinite"] == 0 and h["first_nonfinite"] == 0xFFFFFFFF and h["flags"] == 0
+            assert h["sort_error"] == 0
             stats = b.read_stats()
             assert np.float32(h["max_speed"]).view(np.uint32) == stats[0]
         for name in FIELDS:
