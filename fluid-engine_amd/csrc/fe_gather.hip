@@ -159,6 +159,8 @@ struct BuildStage {
     float4 pos[kCand];   // candidate positions (hash order position_b)
     uint32_t idx[kCand]; // hash index, then schedule position of each candidate
 };
+// A test round reads pos[q] for q < kCand + 63, which stays inside the stage.
+static_assert(sizeof(BuildStage) >= (kCand + 63) * sizeof(float4), "stage read bound");
 
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
 
@@ -193,8 +195,9 @@ __device__ void build_group(const Sph& s, const uint2* __restrict__ range, const
         }
     }
     const uint32_t boff = wave_excl_scan(bcnt);
-    const uint32_t C = __shfl(boff + bcnt, 26, 64);
-    const uint32_t np = ge - gs;
+    // wave-uniform loop bounds (scalar registers)
+    const uint32_t C = __builtin_amdgcn_readfirstlane(__shfl(boff + bcnt, 26, 64));
+    const uint32_t np = __builtin_amdgcn_readfirstlane(ge - gs);
     uint32_t S = 64;
     while (S > 1 && S * np > 64u) S >>= 1;
     const uint32_t per_chunk = 64u / S;
@@ -208,8 +211,10 @@ __device__ void build_group(const Sph& s, const uint2* __restrict__ range, const
         const uint32_t tp = pb + (uint32_t)lane / S;
         const bool act = tp < ge;
         const float4 pi = pos_m[act ? tp : gs];
-        // tp's row of the tiled list: slot sl is at ((sl / 4) 64 + sl % 4) from it
-        uint32_t* const row = nbr + ((((size_t)(tp >> 4) * (kMaxNbr / 4)) << 6) + ((tp & 15u) << 2));
+        // tp's row, as a byte offset from the tile of pb (tp - pb < 64); slot sl sits
+        // (sl / 4) 256 + (sl % 4) 4 bytes from the row
+        char* const tiles = reinterpret_cast<char*>(nbr + (size_t)(pb >> 4) * (kMaxNbr / 4) * 64u);
+        const uint32_t row = ((((tp >> 4) - (pb >> 4)) * (kMaxNbr / 4)) << 8) + ((tp & 15u) << 4);
         uint32_t m = 0;
         for (uint32_t cb = 0; cb < C; cb += kCand) {
             const uint32_t cn = (C - cb) < kCand ? (C - cb) : kCand;
@@ -230,19 +235,24 @@ __device__ void build_group(const Sph& s, const uint2* __restrict__ range, const
                 staged = cb;
             }
             for (uint32_t r = 0; r < cn; r += S) {
+                // q < kCand + 63: past the chunk's end (q >= cn) a lane reads other words of
+                // its wave's stage, and rejects them
                 const uint32_t q = r + sub;
-                bool ok = false;
-                if (act && q < cn) {
-                    const float4 pj = st.pos[q];
-                    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                    ok = !(rx * rx + ry * ry + rz * rz > s.hh);
-                }
+                const float4 pj = st.pos[q];
+                const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                const bool ok = !((act & (sub < cn - r) ? r2 : __builtin_inff()) > s.hh);
                 // the particle's accepted candidates, numbered in lane order
-                const uint64_t seg = __ballot(ok) & segabs;
+                const uint64_t seg = __builtin_amdgcn_ballot_w64(ok) & segabs;
                 if (ok) {
-                    const uint32_t slot = m + __builtin_amdgcn_mbcnt_hi((uint32_t)(seg >> 32),
-                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)seg, 0u));
-                    if (slot < kMaxNbr) row[((slot >> 2) << 6) + (slot & 3u)] = st.idx[q];
+                    uint32_t slot = m + __builtin_amdgcn_mbcnt_hi((uint32_t)(seg >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)seg, 0u));
+                    // a particle with more than kMaxNbr pairs is swept by the bucket walk,
+                    // which never reads its row: its extra pairs may land on its last slot
+                    slot = slot < kMaxNbr ? slot : kMaxNbr - 1u;
+                    // (slot / 4) 256 + (slot % 4) 4 = 4 slot + 240 (slot / 4)
+                    const uint32_t off = row + (slot << 2) + (slot >> 2) * 240u;
+                    *reinterpret_cast<uint32_t*>(tiles + off) = st.idx[q];
                 }
                 m += (uint32_t)__popcll(seg);
             }
@@ -258,7 +268,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_build_list(Sph s0, Scenes sc, u
                                                        const uint32_t* __restrict__ inv, uint32_t* __restrict__ nbr,
                                                        uint32_t* __restrict__ cnt) {
     __shared__ BuildStage stage[kWaves];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t base = (xcd_block() * kWaves + w) * 64u;
     if (base >= n) return;
     const uint32_t t = base + lane;
