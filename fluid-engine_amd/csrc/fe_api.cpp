@@ -454,14 +454,15 @@ fe_status neighbor_search(fe_ctx* c) {
     } else {
         // Real hashes are < T and the T - n sentinels stay in place under a stable sort,
         // so sorting the first n entries on log2(T) bits gives the reference permutation.
-        FE_HIP(fe::radix_sort(c, c->n, c->log2T), "radix sort");
+        // the state is in the previous search's hash order: the new hashes arrive almost sorted
+        FE_HIP(fe::radix_sort(c, c->n, c->log2T, true), "radix sort");
     }
     FE_HIP(fe::launch_offsets_reorder(c), "offsets/reorder");
     c->ends_dirty = false;
     c->list_valid = false;
     c->ws.feat_fresh = false;
     // gather schedule: particles in Morton order of their cells (keys from offsets/reorder)
-    FE_HIP(fe::radix_sort_pairs(c, c->mkeys, c->sched, c->n, c->morton_bits + c->scene_bits, "schedule"),
+    FE_HIP(fe::radix_sort_pairs(c, c->mkeys, c->sched, c->n, c->morton_bits + c->scene_bits, false),
            "schedule sort");
     FE_HIP(fe::launch_schedule_mirrors(c), "schedule mirrors");
     c->ns_fresh = true;
@@ -1018,7 +1019,7 @@ FE_STAGE(fe_sort, {
     if (c->options.sort_algorithm == FE_SORT_BITONIC)
         FE_HIP(fe::bitonic_sort(c), "bitonic sort");
     else
-        FE_HIP(fe::radix_sort(c, c->T, 32), "radix sort");
+        FE_HIP(fe::radix_sort(c, c->T, 32, false), "radix sort");
     c->ends_dirty = true;
     c->ns_fresh = false;
     c->da_fresh = false;

@@ -329,17 +329,20 @@ struct Launch {
 
 // ---- launchers (fe_sort.hip) ----
 // Stable LSD radix sort of n (key, value) pairs on bits [0, key_bits) of the key.
-// The result is in keys[0]/vals[0] on return (buffers are swapped when needed).
-hipError_t radix_sort(fe_ctx* c, uint32_t n, uint32_t key_bits);
+// The result is in keys[0]/vals[0] on return (buffers are swapped when needed). runs: the
+// keys arrive almost sorted, so the digit counts add runs of equal digits (any order sorts
+// correctly either way; this only picks the faster count).
+hipError_t radix_sort(fe_ctx* c, uint32_t n, uint32_t key_bits, bool runs);
 // The same sort on an explicit ping-pong pair of buffers (result in keys[0]/vals[0]).
 hipError_t radix_sort_pairs(fe_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
-                            const char* tag);
+                            bool runs);
 // The bitonic network of bitonic_sort.comp over all T entries, in place in keys[0]/vals[0].
 hipError_t bitonic_sort(fe_ctx* c);
 // Stable LSD radix sort of n pairs on bits [0, key_bits) on `stream` with explicit scratch;
-// result in keys[0]/vals[0] (buffers swapped as needed). prof (may be null) records launches.
+// result in keys[0]/vals[0] (buffers swapped as needed). prof (may be null) records launches;
+// runs as for radix_sort.
 hipError_t radix_sort_core(const SortScratch& s, hipStream_t stream, uint32_t* keys[2], uint32_t* vals[2],
-                           uint32_t n, uint32_t key_bits, fe_ctx* prof);
+                           uint32_t n, uint32_t key_bits, fe_ctx* prof, bool runs);
 // Tiles of a sort of max_n keys, and the look-back words a scratch for that many tiles holds.
 uint32_t sort_max_tiles(uint32_t max_n);
 size_t sort_status_words(uint32_t max_tiles);

@@ -85,10 +85,14 @@ __device__ __forceinline__ void lb_store(uint32_t* p, uint32_t v) {
 }
 
 // Digit counts of every pass over keys[0, n), each block over chunks of kHt x kHistItems keys
-// (all loads of a thread in flight before its LDS counts). Also clears the look-back words of
-// every pass and resets the tile tickets, ahead of the scatter launches that follow on the
-// stream. hist[] must be zero on entry (the previous sort's last tile leaves it so).
-template <int kHt>
+// (all loads of a thread in flight before its LDS counts). kRuns: each thread counts
+// kHistItems consecutive keys and adds each run of one digit with a single LDS atomic, which
+// suits keys that arrive almost sorted (a hash-ordered state's new cell hashes, where whole
+// cells keep their hash); otherwise the threads stride over the chunk, one atomic per key.
+// Also clears the look-back words of every pass and resets the tile tickets, ahead of the
+// scatter launches that follow on the stream. hist[] must be zero on entry (the previous
+// sort's last tile leaves it so).
+template <int kHt, bool kRuns>
 __global__ __launch_bounds__(kHt) void k_onesweep_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t passes,
                                                        uint32_t status_words, uint32_t* __restrict__ hist,
                                                        uint32_t* __restrict__ status, uint32_t* __restrict__ ctrl) {
@@ -103,15 +107,52 @@ __global__ __launch_bounds__(kHt) void k_onesweep_hist(const uint32_t* __restric
     const uint32_t chunks = (n + kChunk - 1) / kChunk;
     for (uint32_t t = blockIdx.x; t < chunks; t += gridDim.x) {
         uint32_t k[kHistItems];
+        if constexpr (kRuns) {
+            const uint32_t b = t * kChunk + tid * kHistItems;
+            if (b + kHistItems <= n) {
 #pragma unroll
-        for (int r = 0; r < kHistItems; r++) {
-            const uint32_t i = t * kChunk + r * kHt + tid;
-            k[r] = i < n ? keys[i] : 0u;
-        }
+                for (int r = 0; r < kHistItems; r += 4) {
+                    const uint4 v = *reinterpret_cast<const uint4*>(keys + b + r);   // b is a multiple of 16
+                    k[r] = v.x;
+                    k[r + 1] = v.y;
+                    k[r + 2] = v.z;
+                    k[r + 3] = v.w;
+                }
+            } else {
 #pragma unroll
-        for (int r = 0; r < kHistItems; r++) {
-            if (t * kChunk + r * kHt + tid < n)
-                for (uint32_t p = 0; p < passes; p++) atomicAdd(&h[p][(k[r] >> (8u * p)) & 0xFFu], 1u);
+                for (int r = 0; r < kHistItems; r++) k[r] = b + r < n ? keys[b + r] : 0u;
+            }
+            const uint32_t m = b < n ? (n - b < (uint32_t)kHistItems ? n - b : (uint32_t)kHistItems) : 0u;
+            if (m) {
+                for (uint32_t p = 0; p < passes; p++) {
+                    const uint32_t sh = 8u * p;
+                    uint32_t cur = (k[0] >> sh) & 0xFFu, c = 0;
+#pragma unroll
+                    for (int r = 0; r < kHistItems; r++) {
+                        if ((uint32_t)r < m) {
+                            const uint32_t d = (k[r] >> sh) & 0xFFu;
+                            if (d != cur) {
+                                atomicAdd(&h[p][cur], c);
+                                cur = d;
+                                c = 0;
+                            }
+                            c++;
+                        }
+                    }
+                    atomicAdd(&h[p][cur], c);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < kHistItems; r++) {
+                const uint32_t i = t * kChunk + r * kHt + tid;
+                k[r] = i < n ? keys[i] : 0u;
+            }
+#pragma unroll
+            for (int r = 0; r < kHistItems; r++) {
+                if (t * kChunk + r * kHt + tid < n)
+                    for (uint32_t p = 0; p < passes; p++) atomicAdd(&h[p][(k[r] >> (8u * p)) & 0xFFu], 1u);
+            }
         }
     }
     __syncthreads();
@@ -451,12 +492,13 @@ struct ScatterCfg {
     uint32_t threads, items;
 };
 ScatterCfg scatter_cfg(uint32_t n) { return n <= (1u << 21) ? ScatterCfg{512, 8} : ScatterCfg{256, 16}; }
-constexpr uint32_t kHistThreads = 512;
+// 256 threads x 16 keys per chunk: a 2^20-key sort counts in 256 blocks, one per CU
+constexpr uint32_t kHistThreads = 256;
 constexpr uint32_t kHistBlocks = 512;
 }  // namespace
 
 hipError_t radix_sort_core(const SortScratch& s, hipStream_t stream, uint32_t* keys[2], uint32_t* vals[2],
-                           uint32_t n, uint32_t key_bits, fe_ctx* prof) {
+                           uint32_t n, uint32_t key_bits, fe_ctx* prof, bool runs) {
     if (n <= 1 || key_bits == 0) return hipSuccess;
     if (n > kSortMaxKeys) return hipErrorInvalidValue;
     const ScatterCfg cfg = scatter_cfg(n);
@@ -474,8 +516,12 @@ hipError_t radix_sort_core(const SortScratch& s, hipStream_t stream, uint32_t* k
     const uint32_t words = passes * (tiles + groups) * 256u;
     {
         SortLaunch L(prof, "radix_hist");
-hipLaunchKernelGGL(k_onesweep_hist<kHistThreads>, dim3(hblocks), dim3(kHistThreads), 0, stream, keys[0], n, passes,
-                           words, s.hist, s.status, s.ctrl);
+        if (runs)
+            hipLaunchKernelGGL((k_onesweep_hist<kHistThreads, true>), dim3(hblocks), dim3(kHistThreads), 0, stream,
+                               keys[0], n, passes, words, s.hist, s.status, s.ctrl);
+        else
+            hipLaunchKernelGGL((k_onesweep_hist<kHistThreads, false>), dim3(hblocks), dim3(kHistThreads), 0, stream,
+                               keys[0], n, passes, words, s.hist, s.status, s.ctrl);
     }
     for (uint32_t p = 0; p < passes; p++) {
         {
@@ -494,10 +540,9 @@ hipLaunchKernelGGL(k_onesweep_hist<kHistThreads>, dim3(hblocks), dim3(kHistThrea
 }
 
 hipError_t radix_sort_pairs(fe_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
-                            const char* tag) {
-    (void)tag;
+                            bool runs) {
     if (n <= 1 || key_bits == 0) return hipSuccess;
-    hipError_t e = radix_sort_core(c->sort, c->stream, keys, vals, n, key_bits, c);
+    hipError_t e = radix_sort_core(c->sort, c->stream, keys, vals, n, key_bits, c, runs);
     if (e != hipSuccess) return e;
     const uint32_t passes = key_bits >= 8 * kSortMaxPasses ? kSortMaxPasses : (key_bits + 7) / 8;
     if ((passes & 1u) && c->batch_graph) {
@@ -512,8 +557,8 @@ hipError_t radix_sort_pairs(fe_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uin
     return hipGetLastError();
 }
 
-hipError_t radix_sort(fe_ctx* c, uint32_t n, uint32_t key_bits) {
-    return radix_sort_pairs(c, c->keys, c->vals, n, key_bits, "entries");
+hipError_t radix_sort(fe_ctx* c, uint32_t n, uint32_t key_bits, bool runs) {
+    return radix_sort_pairs(c, c->keys, c->vals, n, key_bits, runs);
 }
 
 hipError_t bitonic_sort(fe_ctx* c) {
@@ -621,7 +666,7 @@ extern "C" fe_status fe_sort_pairs_noalloc(int device, uint32_t* entries, uint32
     hipLaunchKernelGGL(k_split_pairs, dim3(blocks), dim3(256), 0, st, reinterpret_cast<const uint2*>(entries), keys[0],
                        vals[0], n);
     FE_HIP(hipGetLastError(), "fe_sort_pairs_noalloc split");
-    FE_HIP(radix_sort_core(s, st, keys, vals, n, 32, nullptr), "fe_sort_pairs_noalloc sort");
+    FE_HIP(radix_sort_core(s, st, keys, vals, n, 32, nullptr, false), "fe_sort_pairs_noalloc sort");
     hipLaunchKernelGGL(k_join_pairs, dim3(blocks), dim3(256), 0, st, keys[0], vals[0], reinterpret_cast<uint2*>(entries),
                        n);
     FE_HIP(hipGetLastError(), "fe_sort_pairs_noalloc join");
