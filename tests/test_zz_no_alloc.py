@@ -93,7 +93,7 @@ def test_noalloc_sort_equals_fe_sort_twice(hip):
 def test_noalloc_sort_ragged_and_limits(hip):
     lib = fe.load_library()
     rng = np.random.default_rng(7)
-    for n in (100003, 1, 2, 4097):
+    for n in (100003, 1, 2, 4097, 2500003):   # the last: 512 x 16-key tiles, a partial one at the end
         keys = rng.integers(0, 1 << 20, size=n, dtype=np.uint64).astype(np.uint32)
         pairs = np.stack([keys, rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)], axis=1)
         got = hip.sort(lib, pairs)
