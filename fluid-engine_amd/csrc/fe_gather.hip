@@ -1213,9 +1213,11 @@ __device__ __forceinline__ void sweep_epilogue(unsigned long long q, uint32_t sc
     }
 }
 
-// One work-item's gather: the listed pairs of schedule position t, four neighbour loads in
-// flight, or the bucket walk when the list is stale or the particle overflowed it.
-template <class P>
+// One work-item's gather: the listed pairs of schedule position t, kDepth neighbour loads in
+// flight (4, or 16 for small scenes, where a gather is a chain of dependent list and
+// neighbour loads rather than a stream), or the bucket walk when the list is stale or the
+// particle overflowed it. Either depth visits the pairs in list order.
+template <class P, int kDepth = 4>
 __device__ __forceinline__ unsigned long long gather_one(const P& pol, const Sph& s, uint32_t t,
                                                          const uint2* __restrict__ range,
                                                          const float4* __restrict__ pos_b,
@@ -1230,19 +1232,32 @@ __device__ __forceinline__ unsigned long long gather_one(const P& pol, const Sph
     typename P::Acc acc = pol.zero();
     const uint32_t m = list_ok ? cnt[t] : 0xFFFFFFFFu;
     if (m <= kMaxNbr) {
+        static_assert(kDepth % 4 == 0 && kMaxNbr % kDepth == 0, "whole list chunks");
         const uint4* tile = list_tile(nbr, t);
-        for (uint32_t k = 0; k < m; k += 4) {
-            const uint4 q4 = tile[(k >> 2) * 16];
-            const uint32_t jt[4] = {q4.x, k + 1 < m ? q4.y : q4.x, k + 2 < m ? q4.z : q4.x, k + 3 < m ? q4.w : q4.x};
-            float4 pj[4];
-            typename P::Cand cj[4];
+        for (uint32_t k = 0; k < m; k += kDepth) {
+            // the row's chunks k / 4 .. k / 4 + kDepth / 4 - 1 (inside its kMaxNbr slots); slots
+            // at m and beyond load the batch's first neighbour again and add nothing
+            uint4 q4[kDepth / 4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int b = 0; b < kDepth / 4; b++) q4[b] = tile[((k >> 2) + b) * 16];
+            uint32_t jt[kDepth];
+#pragma unroll
+            for (int b = 0; b < kDepth / 4; b++) {
+                const uint32_t k0 = k + 4 * b;
+                jt[4 * b] = b == 0 || k0 < m ? q4[b].x : q4[0].x;
+                jt[4 * b + 1] = k0 + 1 < m ? q4[b].y : q4[0].x;
+                jt[4 * b + 2] = k0 + 2 < m ? q4[b].z : q4[0].x;
+                jt[4 * b + 3] = k0 + 3 < m ? q4[b].w : q4[0].x;
+            }
+            float4 pj[kDepth];
+            typename P::Cand cj[kDepth];
+#pragma unroll
+            for (int u = 0; u < kDepth; u++) {
                 pj[u] = geo_of(pol, pos_m, jt[u]);
                 cj[u] = pol.load(jt[u], pj[u]);
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < kDepth; u++)
                 if (k + u < m) pol.pair(acc, si, s, pi.x - pj[u].x, pi.y - pj[u].y, pi.z - pj[u].z, cj[u]);
         }
     } else {
@@ -1259,7 +1274,7 @@ __device__ __forceinline__ unsigned long long gather_one(const P& pol, const Sph
 // constants are kernel arguments, and the sweep returns at once when the previous sweep
 // converged. A batch: each work-item takes its scene's constants from the table and skips
 // when its own scene converged (a batch's slots are kMaxSolveIters apart per scene).
-template <class P, bool kBatch, int B>
+template <class P, bool kBatch, int B, int kDepth = 4>
 __global__ __launch_bounds__(B) void k_list_gather(P pol, Sph s0, Scenes sc, uint32_t n,
                                                         const uint2* __restrict__ range,
                                                         const float4* __restrict__ pos_b,
@@ -1282,7 +1297,7 @@ __global__ __launch_bounds__(B) void k_list_gather(P pol, Sph s0, Scenes sc, uin
             if (!(prev && prev[(size_t)scene * kMaxSolveIters] <= (kind ? s.thr[1] : s.thr[0])))
                 q = gather_one(pol, s, t, range, pos_b, pos_m, sched, inv, nbr, cnt, list_ok);
         } else {
-            q = gather_one(pol, s0, t, range, pos_b, pos_m, sched, inv, nbr, cnt, list_ok);
+            q = gather_one<P, kDepth>(pol, s0, t, range, pos_b, pos_m, sched, inv, nbr, cnt, list_ok);
         }
     }
     sweep_epilogue<kBatch, B>(q, scene, slot_out);
@@ -1417,17 +1432,28 @@ hipError_t need(fe_ctx* c, uint32_t bits) {
     return hipGetLastError();
 }
 
-template <class P, bool kBatch, int B>
+template <class P, bool kBatch, int B, int kDepth = 4>
 void launch_gather(fe_ctx* c, const P& pol, const unsigned long long* prev,
                    unsigned long long thr, uint32_t kind, unsigned long long* slot_out) {
-    hipLaunchKernelGGL((k_list_gather<P, kBatch, B>), dim3((c->n + B - 1) / B), dim3(B), 0, c->stream, pol,
+    hipLaunchKernelGGL((k_list_gather<P, kBatch, B, kDepth>), dim3((c->n + B - 1) / B), dim3(B), 0, c->stream, pol,
                        sph_of(c), scenes_of(c), c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr,
                        c->nbr_count, c->list_valid ? 1u : 0u, prev, thr, kind, slot_out);
 }
 
+// items 4 (one scene, blocks of 64 to 256, no wall samples): 16 neighbour loads in flight
+// per work-item
 template <class P, bool kBatch>
 void launch_gather_b(fe_ctx* c, uint32_t block, const P& pol, const unsigned long long* prev, unsigned long long thr, uint32_t kind,
-                     unsigned long long* slot_out) {
+                     unsigned long long* slot_out, uint32_t items = 1) {
+    if constexpr (!kBatch && !P::kWalls) {
+        if (items == 4) {
+            switch (block) {
+                case 64: launch_gather<P, false, 64, 16>(c, pol, prev, thr, kind, slot_out); return;
+                case 128: launch_gather<P, false, 128, 16>(c, pol, prev, thr, kind, slot_out); return;
+                default: launch_gather<P, false, 256, 16>(c, pol, prev, thr, kind, slot_out); return;
+            }
+        }
+    }
     switch (block) {
         case 64: launch_gather<P, kBatch, 64>(c, pol, prev, thr, kind, slot_out); break;
         case 128: launch_gather<P, kBatch, 128>(c, pol, prev, thr, kind, slot_out); break;
@@ -1441,11 +1467,11 @@ template <class P>
 hipError_t run_list(fe_ctx* c, const char* name, int tid, const P& pol, const unsigned long long* prev = nullptr, unsigned long long thr = 0, uint32_t kind = 0,
                     unsigned long long* slot_out = nullptr) {
     Launch L(c, name);
-    const uint32_t block = c->tune[tid].block;
+    const uint32_t block = tid >= 0 ? c->tune[tid].block : 256u, items = tid >= 0 ? c->tune[tid].items : 1u;
     if (c->nscenes > 1)
         launch_gather_b<P, true>(c, block, pol, prev, thr, kind, slot_out);
     else
-        launch_gather_b<P, false>(c, block, pol, prev, thr, kind, slot_out);
+        launch_gather_b<P, false>(c, block, pol, prev, thr, kind, slot_out, items);
     return hipGetLastError();
 }
 

@@ -779,11 +779,13 @@ bool tune_valid(int id, uint32_t block, uint32_t items) {
     if (id == kTuneBuildList) return (block == 64 || block == 128 || block == 256) && items == 1;
     if (id == kTunePressureIntegration || id == kTuneDivergenceIntegration)
         return pow2 && (items == 1 || items == 2 || items == 4);
-    // the sweeps: items 1 gathers each listed pair from the mirrors, items 2 stages one window
-    // per wave in LDS (1, 2 or 4 windows per block)
+    // the list gathers: items 1 gathers each listed pair from the mirrors with 4 loads in
+    // flight, items 4 with 16 (one scene, blocks of 64 to 256); for the sweeps items 2 stages
+    // one window per wave in LDS (1, 2 or 4 windows per block)
+    const bool small = block == 64 || block == 128 || block == 256;
     if (id == kTunePressureForce || id == kTunePressureUpdate)
-        return (pow2 && items == 1) || ((block == 64 || block == 128 || block == 256) && items == 2);
-    return id >= 0 && id < kTuneCount && pow2 && items == 1;
+        return (pow2 && items == 1) || (small && (items == 2 || items == 4));
+    return id >= 0 && id < kTuneCount && ((pow2 && items == 1) || (small && items == 4));
 }
 
 hipError_t launch_unpack_start(fe_ctx* c, const uint32_t* in) {

@@ -332,10 +332,13 @@ fe_status fe_get_wall(const fe_ctx* ctx, fe_wall* out, int* attached);
  *   divergence_source                fe_divergence_source only
  *   viscosity, density_source, pressure_force, pressure_update, build_list and the two
  *   integrations                     every launch of that kernel. 
- * For the two Jacobi sweeps, items selects the variant: 1 gathers each listed pair from the
- * schedule-ordered mirrors; 2 stages each 64-particle window's neighbour union in LDS and
- * walks 16-bit lists (block 64, 128 or 256 = 1, 2 or 4 windows per block). fe_autotune
- * tunes variant 1 only (variant 2 builds window tables once per list, which timing sweep
+ * For the seven list gathers, items selects the variant: 1 gathers each listed pair from the
+ * schedule-ordered mirrors with 4 neighbour records in flight per work-item; 4 does the same
+ * with 16 in flight (block 64, 128 or 256; single-scene contexts without wall samples, a
+ * batch or the wall mode runs variant 1), which shortens a small scene's chain of dependent
+ * loads; for the two Jacobi sweeps, 2 stages each 64-particle window's neighbour union in LDS
+ * and walks 16-bit lists (block 64, 128 or 256 = 1, 2 or 4 windows per block). fe_autotune
+ * tunes variants 1 and 4 (variant 2 builds window tables once per list, which timing sweep
  * launches alone would not count); fe_tune_set selects variant 2.
  * fe_autotune times every configuration on the context's current state (the median of
  * `reps` launches, 0 = 20) and keeps the fastest; the fields the timed launches overwrite

@@ -1,0 +1,150 @@
+"""The deep list gathers (tune items 4: 16 listed neighbours' records in flight per
+work-item instead of 4, one scene, blocks of 64 to 256) on the GPU, through the C ABI. They
+visit every particle's pairs in the same list order, so fields, grid tables, stats and sweep
+counts equal those of the default gathers (items 1) bit for bit, and the oracle's directly.
+Covered: fixed and converging frames at every deep block size, the reference app's scene
+with its moving wall replayed from a graph, particles that overflow the list (bucket walk),
+and a batch (which keeps the four-deep gathers)."""
+import numpy as np
+import pytest
+
+import fluid_engine as fe
+from fluid_engine import scene
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("position_a", "velocity_a", "velocity_b", "position_b", "densities", "factors", "source_terms",
+          "pressures", "pressure_accelerations", "colors")
+GATHERS = ("density_alpha", "viscosity", "density_source", "divergence_source", "pressure_force", "pressure_update",
+           "density_alpha_divergence_source")
+
+
+def opts(mode):
+    o = fe.default_options()
+    o.sort_algorithm = fe.FE_SORT_RADIX
+    o.solver_mode = mode
+    o.density_tol = 1e-3
+    o.divergence_tol = 1e-2
+    o.max_iterations = 100
+    return o
+
+
+def same(name, a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, name
+    va = a.view(np.uint32) if a.dtype == np.float32 else a
+    vb = b.view(np.uint32) if b.dtype == np.float32 else b
+    if not np.array_equal(va, vb):
+        bad = np.nonzero(va.reshape(-1) != vb.reshape(-1))[0]
+        raise AssertionError(f"{name}: {len(bad)} of {va.size} differ, first at {bad[:5].tolist()}")
+
+
+def deep(sim, block):
+    for k in GATHERS:
+        sim.set_tune(k, block, 4)
+
+
+def compare(a, b, label):
+    for name in FIELDS + ("grid_entries", "grid_start"):
+        same(f"{label} {name}", a.download(name), b.download(name))
+    same(f"{label} stats", a.read_stats(), b.read_stats())
+    assert a.iteration_totals() == b.iteration_totals(), label
+
+
+@pytest.mark.parametrize("mode", [fe.FE_SOLVER_FIXED, fe.FE_SOLVER_CONVERGE])
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_deep_matches_default_32k(mode, block):
+    w = scene.dam_break_32k(seed=9)
+    with fe.Simulation(w.params, w.positions, device=0, options=opts(mode)) as a, \
+            fe.Simulation(w.params, w.positions, device=0, options=opts(mode)) as b:
+        deep(b, block)
+        assert all(e["items"] == 4 and e["block"] == block for k, e in b.tune_entries().items() if k in GATHERS)
+        a.run(5)
+        b.run(5)
+        compare(a, b, f"mode {mode} block {block}")
+
+
+def test_deep_matches_oracle(oracle_mod):
+    w = scene.rest_density_scene()
+    o = opts(fe.FE_SOLVER_FIXED)
+    with fe.Simulation(w.params, w.positions, device=0, options=o) as s:
+        deep(s, 128)
+        orc = oracle_mod.Oracle(w.params, w.positions, options=o)
+        try:
+            for f in range(3):
+                s.step()
+                orc.step()
+                for name in FIELDS:
+                    same(f"frame {f} {name} vs oracle", s.download(name), orc.download(name))
+                same(f"frame {f} stats vs oracle", s.read_stats(), orc.read_stats())
+        finally:
+            orc.close()
+
+
+def test_deep_app_scene_graph_frames():
+    w = scene.app_scene()
+    o = opts(fe.FE_SOLVER_FIXED)
+    sims = []
+    try:
+        for _ in range(2):
+            s = fe.Simulation(w.params, w.positions, device=0, options=o)
+            s.set_wall(fe.CollisionBox.from_params(w.params, *w.wall_wave))
+            sims.append(s)
+        a, b = sims
+        deep(b, 64)
+        b.use_graph(True)
+        for f in range(6):
+            a.step()
+            b.step()
+            compare(a, b, f"app frame {f}")
+        a.run(30)
+        b.run(30)
+        compare(a, b, "app after 36 frames")
+    finally:
+        for s in sims:
+            s.close()
+
+
+def test_deep_overflowing_lists_take_the_walk(oracle_mod):
+    """A clump of 300 particles in half a smoothing length (about 299 pairs each, past the
+    list's 64) beside ordinary particles: the clump walks the buckets, the rest read their
+    lists 16 at a time, stage by stage against the oracle."""
+    rng = np.random.default_rng(5)
+    w = scene.dam_break_32k(seed=4)
+    pos = w.positions.copy()
+    pos[:300] = np.array([0.5, 0.3, 0.5], np.float32) + rng.uniform(0, 0.02, (300, 3)).astype(np.float32)
+    o = opts(fe.FE_SOLVER_FIXED)
+    with fe.Simulation(w.params, pos, device=0, options=o) as s:
+        deep(s, 256)
+        orc = oracle_mod.Oracle(w.params, pos, options=o)
+        try:
+            for x in (s, orc):
+                x.neighbor_search()
+                x.density_alpha()
+                x.viscosity()
+                x.density_source()
+                x.pressure_force()
+                x.pressure_update()
+            for name in ("densities", "factors", "velocity_a", "source_terms", "pressure_accelerations", "pressures"):
+                same("clump " + name, s.download(name), orc.download(name))
+        finally:
+            orc.close()
+
+
+def test_batch_keeps_four_deep_gathers():
+    ws = scene.batch_scenes(count=4, seed=3, n_min=3000, n_max=12000)
+    o = opts(fe.FE_SOLVER_CONVERGE)
+    a = fe.Batch(ws, device=0, options=o)
+    b = fe.Batch(ws, device=0, options=o)
+    try:
+        deep(b, 128)
+        a.run(3)
+        b.run(3)
+        for name in FIELDS:
+            same(name, a.download(name), b.download(name))
+        for k in range(len(ws)):
+            assert a.scene_iterations(k) == b.scene_iterations(k), k
+    finally:
+        a.close()
+        b.close()
