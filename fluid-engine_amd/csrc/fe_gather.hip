@@ -261,15 +261,20 @@ __device__ void build_group(const Sph& s, const uint2* __restrict__ range, const
     }
 }
 
-template <bool kBatch, int kWaves>
+// A wave takes the cell groups that start in its window of kWin consecutive schedule
+// positions (each group runs on past the window's end if its cell does). kWin 64 (tune items
+// 1): about eight groups one after another per wave; kWin 8 (items 8): about one, so a small
+// scene's groups are built side by side instead of in a chain.
+template <bool kBatch, int kWaves, int kWin = 64>
 __global__ __launch_bounds__(64 * kWaves) void k_build_list(Sph s0, Scenes sc, uint32_t n, const uint2* __restrict__ range,
                                                        const float4* __restrict__ pos_b,
                                                        const float4* __restrict__ pos_m,
                                                        const uint32_t* __restrict__ inv, uint32_t* __restrict__ nbr,
                                                        uint32_t* __restrict__ cnt) {
+    static_assert(kWin == 64 || kWin == 8, "window");
     __shared__ BuildStage stage[kWaves];
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t base = (xcd_block() * kWaves + w) * 64u;
+    const uint32_t base = (xcd_block() * kWaves + w) * (uint32_t)kWin;
     if (base >= n) return;
     const uint32_t t = base + lane;
     const bool valid = t < n;
@@ -285,7 +290,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_build_list(Sph s0, Scenes sc, u
         sp = kBatch ? sc.sid[base - 1] : 0u;
         cp = cell_of(q.x, q.y, q.z, kBatch ? sc.tab[sp].inv_h : s0.inv_h);
     }
-    const bool start = valid && (t == 0 || c.x != cp.x || c.y != cp.y || c.z != cp.z || sid != sp);
+    const bool start = valid && lane < kWin && (t == 0 || c.x != cp.x || c.y != cp.y || c.z != cp.z || sid != sp);
     uint64_t starts = __ballot(start);
     while (starts) {
         const int sl = __builtin_ctzll(starts);
@@ -299,7 +304,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_build_list(Sph s0, Scenes sc, u
             ge = base + (uint32_t)__builtin_ctzll(starts);
         } else {
             // the group runs to the end of the window, and possibly beyond it
-            ge = base + 64u < n ? base + 64u : n;
+            ge = base + (uint32_t)kWin < n ? base + (uint32_t)kWin : n;
             while (ge < n) {
                 const uint32_t j = ge + lane;
                 const bool vj = j < n;
@@ -1303,6 +1308,113 @@ __global__ __launch_bounds__(B) void k_list_gather(P pol, Sph s0, Scenes sc, uin
     sweep_epilogue<kBatch, B>(q, scene, slot_out);
 }
 
+// ---- four lanes per particle (tune items 8, the two sweeps, one scene) ----------------------
+// Lane r of a particle's quad (four consecutive lanes) takes the listed slots r, r + 4, r + 8,
+// ..., loads kQuadDepth of them at a time and forms each slot's term from a zero
+// accumulator (0 + term = term, and a sum that starts at +0 never holds -0, so adding the
+// term alone is adding the pair). The quad then adds the four terms of every group of four
+// slots in slot order through DPP broadcasts, so each lane holds the particle's sums exactly
+// as the one-lane gather forms them; lane 0 finishes the particle. A small scene's sweep then
+// runs four times as many waves, each with a quarter of the pair arithmetic in its chain.
+// Policies whose pair() only adds a term to each accumulator component can take it.
+constexpr int kQuadDepth = 8;   // slots per lane loaded at once: 32 per particle
+
+// x of quad lane u, in every lane of the quad
+__device__ __forceinline__ float quad_bcast(float x, int u) {
+    const int ctrl = u | (u << 2) | (u << 4) | (u << 6);   // quad_perm [u, u, u, u]
+    int v;
+    switch (ctrl) {
+        case 0x00: v = __builtin_amdgcn_mov_dpp(__float_as_int(x), 0x00, 0xF, 0xF, false); break;
+        case 0x55: v = __builtin_amdgcn_mov_dpp(__float_as_int(x), 0x55, 0xF, 0xF, false); break;
+        case 0xAA: v = __builtin_amdgcn_mov_dpp(__float_as_int(x), 0xAA, 0xF, 0xF, false); break;
+        default: v = __builtin_amdgcn_mov_dpp(__float_as_int(x), 0xFF, 0xF, 0xF, false); break;
+    }
+    return __int_as_float(v);
+}
+
+// acc += term of quad lane 0, then 1, 2, 3 (component by component).
+template <class Acc>
+__device__ __forceinline__ void quad_add(Acc& acc, const Acc& term) {
+    constexpr int N = sizeof(Acc) / sizeof(float);
+    static_assert(sizeof(Acc) == N * sizeof(float), "float accumulators");
+    float a[N], t[N];
+    __builtin_memcpy(a, &acc, sizeof(Acc));
+    __builtin_memcpy(t, &term, sizeof(Acc));
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int k = 0; k < N; k++) a[k] += quad_bcast(t[k], u);
+    __builtin_memcpy(&acc, a, sizeof(Acc));
+}
+
+template <class P>
+__device__ __forceinline__ unsigned long long gather_quad(const P& pol, const Sph& s, uint32_t t, uint32_t r,
+                                                          const uint2* __restrict__ range,
+                                                          const float4* __restrict__ pos_b, const float4* pos_m,
+                                                          const uint32_t* __restrict__ sched,
+                                                          const uint32_t* __restrict__ inv,
+                                                          const uint32_t* __restrict__ nbr,
+                                                          const uint32_t* __restrict__ cnt, uint32_t list_ok) {
+    const uint32_t i = sched[t];
+    const float4 pi = geo_of(pol, pos_m, t);
+    const typename P::Self si = pol.self(i, t, pi);
+    typename P::Acc acc = pol.zero();
+    const uint32_t m = list_ok ? cnt[t] : 0xFFFFFFFFu;   // the same in the four lanes
+    if (m <= kMaxNbr) {
+        // slot k of t's row: uint32 ((t / 16) (kMaxNbr / 4) + k / 4) 64 + (t % 16) 4 + k % 4
+        const uint32_t* row = nbr + (((size_t)(t >> 4) * (kMaxNbr / 4)) << 6) + ((t & 15u) << 2) + r;
+        for (uint32_t k0 = 0; k0 < m; k0 += 4 * kQuadDepth) {
+            // slots k0 + 4 b + r < kMaxNbr: inside the row; past m the particle's own record
+            // is loaded and nothing is added
+            uint32_t jt[kQuadDepth];
+#pragma unroll
+            for (int b = 0; b < kQuadDepth; b++) {
+                const uint32_t k = k0 + 4 * b;
+                const uint32_t e = row[(k >> 2) << 6];
+                jt[b] = k + r < m ? e : t;
+            }
+            float4 pj[kQuadDepth];
+            typename P::Cand cj[kQuadDepth];
+#pragma unroll
+            for (int b = 0; b < kQuadDepth; b++) {
+                pj[b] = geo_of(pol, pos_m, jt[b]);
+                cj[b] = pol.load(jt[b], pj[b]);
+            }
+#pragma unroll
+            for (int b = 0; b < kQuadDepth; b++) {
+                typename P::Acc term = pol.zero();
+                if (k0 + 4 * b + r < m) pol.pair(term, si, s, pi.x - pj[b].x, pi.y - pj[b].y, pi.z - pj[b].z, cj[b]);
+                quad_add(acc, term);
+            }
+        }
+    } else if (r == 0) {
+        visit27(range, pos_b, s, pi.x, pi.y, pi.z, [&](uint32_t j, const float4& pj) {
+            const uint32_t jt = inv[j];
+            pol.pair(acc, si, s, pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, pol.load(jt, geo_of(pol, pos_m, jt)));
+        });
+    }
+    return r == 0 ? pol.finish(i, t, acc, si, s) : 0ull;
+}
+
+template <class P, int B>
+__global__ __launch_bounds__(B) void k_quad_gather(P pol, Sph s0, uint32_t n, const uint2* __restrict__ range,
+                                                   const float4* __restrict__ pos_b, const float4* pos_m,
+                                                   const uint32_t* __restrict__ sched,
+                                                   const uint32_t* __restrict__ inv,
+                                                   const uint32_t* __restrict__ nbr,
+                                                   const uint32_t* __restrict__ cnt, uint32_t list_ok,
+                                                   const unsigned long long* __restrict__ prev,
+                                                   unsigned long long thr,
+                                                   unsigned long long* __restrict__ slot_out) {
+    static_assert(!P::kWalls, "wall samples take the one-lane gathers");
+    if (prev && *prev <= thr) return;
+    const uint32_t wi = work_item<B>();
+    const uint32_t t = wi >> 2, r = wi & 3u;
+    unsigned long long q = 0;
+    if (t < n) q = gather_quad(pol, s0, t, r, range, pos_b, pos_m, sched, inv, nbr, cnt, list_ok);
+    sweep_epilogue<false, B>(q, 0u, slot_out);
+}
+
 // The staged sweep of one window (one wave): the window's staging slots are loaded from the
 // schedule-ordered mirrors into LDS, then each lane walks its 16-bit list over LDS.
 // run: this lane's particle is swept (in range, its scene not converged).
@@ -1497,9 +1609,27 @@ void launch_win_b(fe_ctx* c, uint32_t block, const P& pol, const unsigned long l
 
 // A Jacobi sweep: staged windows over the current list (tuned items 2), or per-particle
 // gathers of the listed pairs (items 1; the bucket walk when there is no current list).
+template <class P, int B>
+void launch_quad(fe_ctx* c, const P& pol, const unsigned long long* prev, unsigned long long thr,
+                 unsigned long long* slot_out) {
+    const uint64_t items = 4ull * c->n;
+    hipLaunchKernelGGL((k_quad_gather<P, B>), dim3((unsigned)((items + B - 1) / B)), dim3(B), 0, c->stream, pol,
+                       sph_of(c), c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr, c->nbr_count,
+                       c->list_valid ? 1u : 0u, prev, thr, slot_out);
+}
+
 template <class P>
 hipError_t run_sweep(fe_ctx* c, const char* name, int tid, const P& pol, const unsigned long long* prev, unsigned long long thr, uint32_t kind,
                      unsigned long long* slot_out) {
+    if (c->tune[tid].items == 8 && c->nscenes == 1) {   // four lanes per particle
+        Launch L(c, name);
+        switch (c->tune[tid].block) {
+            case 64: launch_quad<P, 64>(c, pol, prev, thr, slot_out); break;
+            case 128: launch_quad<P, 128>(c, pol, prev, thr, slot_out); break;
+            default: launch_quad<P, 256>(c, pol, prev, thr, slot_out); break;
+        }
+        return hipGetLastError();
+    }
     if (!c->list_valid || c->tune[tid].items != 2) return run_list(c, name, tid, pol, prev, thr, kind, slot_out);
     if (c->win_gen != c->list_gen) {   // the current list has no window tables yet
         Launch W(c, "window_tables");
@@ -1545,17 +1675,23 @@ hipError_t launch_build_list(fe_ctx* c) {
     if (e != hipSuccess) return e;
     Launch L(c, "build_list");
     const uint32_t waves = c->tune[kTuneBuildList].block / 64;
-    const dim3 grid((c->n + 64 * waves - 1) / (64 * waves)), blk(64 * waves);
+    const uint32_t win = c->tune[kTuneBuildList].items == 8 ? 8u : 64u;
+    const dim3 grid((c->n + win * waves - 1) / (win * waves)), blk(64 * waves);
     const Sph s0 = sph_of(c);
     const Scenes sc = scenes_of(c);
-#define FE_BUILD(BATCH, W)                                                                                   \
-    hipLaunchKernelGGL((k_build_list<BATCH, W>), grid, blk, 0, c->stream, s0, sc, c->n, c->range, c->pos_b, \
+#define FE_BUILD(BATCH, W, K)                                                                                   \
+    hipLaunchKernelGGL((k_build_list<BATCH, W, K>), grid, blk, 0, c->stream, s0, sc, c->n, c->range, c->pos_b, \
                        c->pos_m, c->inv, c->nbr, c->nbr_count)
+#define FE_BUILD_W(BATCH, K)                                                                \
+    do {                                                                                     \
+        if (waves == 1) FE_BUILD(BATCH, 1, K); else if (waves == 2) FE_BUILD(BATCH, 2, K); else FE_BUILD(BATCH, 4, K); \
+    } while (0)
     if (c->nscenes > 1) {
-        if (waves == 1) FE_BUILD(true, 1); else if (waves == 2) FE_BUILD(true, 2); else FE_BUILD(true, 4);
+        if (win == 8) FE_BUILD_W(true, 8); else FE_BUILD_W(true, 64);
     } else {
-        if (waves == 1) FE_BUILD(false, 1); else if (waves == 2) FE_BUILD(false, 2); else FE_BUILD(false, 4);
+        if (win == 8) FE_BUILD_W(false, 8); else FE_BUILD_W(false, 64);
     }
+#undef FE_BUILD_W
 #undef FE_BUILD
     e = hipGetLastError();
     if (e == hipSuccess) {

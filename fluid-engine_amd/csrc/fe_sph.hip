@@ -776,15 +776,17 @@ const char* tune_name(int id) {
 
 bool tune_valid(int id, uint32_t block, uint32_t items) {
     const bool pow2 = block == 64 || block == 128 || block == 256 || block == 512 || block == 1024;
-    if (id == kTuneBuildList) return (block == 64 || block == 128 || block == 256) && items == 1;
+    // build_list: items 1 = 64 schedule positions per wave, 8 = 8 per wave
+    if (id == kTuneBuildList) return (block == 64 || block == 128 || block == 256) && (items == 1 || items == 8);
     if (id == kTunePressureIntegration || id == kTuneDivergenceIntegration)
         return pow2 && (items == 1 || items == 2 || items == 4);
     // the list gathers: items 1 gathers each listed pair from the mirrors with 4 loads in
     // flight, items 4 with 16 (one scene, blocks of 64 to 256); for the sweeps items 2 stages
     // one window per wave in LDS (1, 2 or 4 windows per block)
     const bool small = block == 64 || block == 128 || block == 256;
+    // (and items 8 runs four lanes per particle, one scene)
     if (id == kTunePressureForce || id == kTunePressureUpdate)
-        return (pow2 && items == 1) || (small && (items == 2 || items == 4));
+        return (pow2 && items == 1) || (small && (items == 2 || items == 4 || items == 8));
     return id >= 0 && id < kTuneCount && ((pow2 && items == 1) || (small && items == 4));
 }
 

@@ -25,7 +25,12 @@ hipError_t launch_tuned(fe_ctx* c, int id) {
         case fe::kTuneDensitySource: return fe::launch_density_source(c);
         case fe::kTuneDivergenceSource: return fe::launch_divergence_source(c);
         case fe::kTunePressureForce: return fe::launch_pressure_force(c, nullptr, 0, 0);
-        case fe::kTunePressureUpdate: return fe::launch_pressure_update(c, nullptr, nullptr, 0, 0);
+        // in converge mode each block of a sweep adds its residual into the sweep's slot, as
+        // in the frame (the slots are cleared when a solve starts): a configuration with many
+        // blocks pays for those same-word atomics here too
+        case fe::kTunePressureUpdate:
+            return fe::launch_pressure_update(c, c->options.solver_mode == FE_SOLVER_CONVERGE ? c->resid : nullptr,
+                                              nullptr, 0, 0);
         case fe::kTuneBuildList: return fe::launch_build_list(c);
         case fe::kTunePressureIntegration: return fe::launch_pressure_integration(c);
         case fe::kTuneDivergenceIntegration: return fe::launch_divergence_integration(c);
@@ -40,7 +45,7 @@ std::vector<fe::TuneCfg> candidates(int id) {
     std::vector<fe::TuneCfg> out;
     const bool sweep = id == fe::kTunePressureForce || id == fe::kTunePressureUpdate;
     for (uint32_t b : {64u, 128u, 256u, 512u, 1024u})
-        for (uint32_t it : {1u, 2u, 4u})
+        for (uint32_t it : {1u, 2u, 4u, 8u})
             if (fe::tune_valid(id, b, it) && !(sweep && it == 2u)) {
                 fe::TuneCfg t;
                 t.block = b;

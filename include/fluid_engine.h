@@ -324,8 +324,9 @@ fe_status fe_set_wall(fe_ctx* ctx, const fe_wall* wall);
 fe_status fe_get_wall(const fe_ctx* ctx, fe_wall* out, int* attached);
 
 /* ---- launch-configuration autotuner (no reference counterpart) --------------------------------
- * Tunable kernels: the seven list gathers (block 64..1024), build_list (block 64, 128, 256)
- * and the two integrations (block 64..1024 x 1, 2 or 4 particles per thread). In fe_step:
+ * Tunable kernels: the seven list gathers (block 64..1024), build_list (block 64, 128, 256;
+ * items 1 = 64 schedule positions per wave, 8 = 8 per wave, for small scenes) and the two
+ * integrations (block 64..1024 x 1, 2 or 4 particles per thread). In fe_step:
  *   density_alpha                    the frame-start density pass (when the last search
  *                                    cannot be reused) and fe_density_alpha
  *   density_alpha_divergence_source  the post-integrate density + divergence-source pass
@@ -337,9 +338,12 @@ fe_status fe_get_wall(const fe_ctx* ctx, fe_wall* out, int* attached);
  * with 16 in flight (block 64, 128 or 256; single-scene contexts without wall samples, a
  * batch or the wall mode runs variant 1), which shortens a small scene's chain of dependent
  * loads; for the two Jacobi sweeps, 2 stages each 64-particle window's neighbour union in LDS
- * and walks 16-bit lists (block 64, 128 or 256 = 1, 2 or 4 windows per block). fe_autotune
- * tunes variants 1 and 4 (variant 2 builds window tables once per list, which timing sweep
- * launches alone would not count); fe_tune_set selects variant 2.
+ * and walks 16-bit lists (block 64, 128 or 256 = 1, 2 or 4 windows per block), and 8 runs four
+ * lanes per particle (block 64, 128 or 256; single-scene contexts, otherwise variant 1), each
+ * forming the terms of every fourth listed pair, added in list order across the four lanes.
+ * fe_autotune tunes variants 1, 4 and 8 (variant 2 builds window tables once per list, which
+ * timing sweep launches alone would not count; fe_tune_set selects it). In converge mode it
+ * times pressure_update with its residual atomics, as the frame runs it.
  * fe_autotune times every configuration on the context's current state (the median of
  * `reps` launches, 0 = 20) and keeps the fastest; the fields the timed launches overwrite
  * are restored, so the next frame is the one the context would have run. Configurations
