@@ -27,6 +27,7 @@
 // the bucket walk, which produces the same pairs in the same order.
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "fe_device.h"
 
@@ -557,6 +558,7 @@ struct DensityAlpha {
     struct Acc { float rho, sgs, gx, gy, gz; };
     static constexpr bool kPaGeo = false;
     static constexpr bool kWalls = false;
+    static constexpr bool kAdditive = true;   // pair() adds one term per Acc component
     __device__ Self self(uint32_t, uint32_t, const float4&) const { return {}; }
     __device__ Cand load(uint32_t, const float4&) const { return {}; }
     __device__ Acc zero() const { return {0.f, 0.f, 0.f, 0.f, 0.f}; }
@@ -616,6 +618,7 @@ struct Viscosity {
     struct Acc { float x, y, z; };
     static constexpr bool kPaGeo = false;
     static constexpr bool kWalls = false;
+    static constexpr bool kAdditive = true;   // pair() adds one term per Acc component
     __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {vb_m[t]}; }
     __device__ Cand load(uint32_t jt, const float4&) const {
         const float4 v = vb_m[jt];
@@ -656,6 +659,7 @@ struct DensitySource {
     struct Acc { float div; };
     static constexpr bool kPaGeo = false;
     static constexpr bool kWalls = false;
+    static constexpr bool kAdditive = true;   // pair() adds one term per Acc component
     __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {va_m[t]}; }
     __device__ Cand load(uint32_t jt, const float4&) const { return {va_m[jt]}; }
     __device__ Acc zero() const { return {0.f}; }
@@ -702,6 +706,7 @@ struct DivergenceSource {
     struct Acc { float div; };
     static constexpr bool kPaGeo = false;
     static constexpr bool kWalls = false;
+    static constexpr bool kAdditive = true;   // pair() adds one term per Acc component
     __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {va_m[t]}; }
     __device__ Cand load(uint32_t jt, const float4&) const { return {va_m[jt]}; }
     __device__ Acc zero() const { return {0.f}; }
@@ -742,6 +747,7 @@ struct DensityAlphaDivergence {
     struct Acc { DensityAlpha::Acc a; float div; };
     static constexpr bool kPaGeo = false;
     static constexpr bool kWalls = false;
+    static constexpr bool kAdditive = true;   // pair() adds one term per Acc component
     __device__ Self self(uint32_t, uint32_t t, const float4&) const { return {dv.va_m[t]}; }
     __device__ Cand load(uint32_t jt, const float4&) const { return {dv.va_m[jt]}; }
     __device__ Acc zero() const { return {da.zero(), 0.f}; }
@@ -963,6 +969,7 @@ struct PressureForce {
     static constexpr int kParts = 1;
     static constexpr bool kPaGeo = false;
     static constexpr bool kWalls = false;
+    static constexpr bool kAdditive = true;   // pair() adds one term per Acc component
     struct Self { float prho, px, py, pz; };
     struct Cand { float prho; };
     struct Acc { float x, y, z; };
@@ -1019,6 +1026,7 @@ struct PressureUpdate {
     static constexpr int kParts = 2;
     static constexpr bool kPaGeo = true;
     static constexpr bool kWalls = false;
+    static constexpr bool kAdditive = true;   // pair() adds one term per Acc component
     struct Self { float ax, ay, az; };
     struct Cand { float ax, ay, az; };
     struct Acc { float ap; };
@@ -1308,7 +1316,7 @@ __global__ __launch_bounds__(B) void k_list_gather(P pol, Sph s0, Scenes sc, uin
     sweep_epilogue<kBatch, B>(q, scene, slot_out);
 }
 
-// ---- four lanes per particle (tune items 8, the two sweeps, one scene) ----------------------
+// ---- four lanes per particle (tune items 8, one scene) ------------------------------------
 // Lane r of a particle's quad (four consecutive lanes) takes the listed slots r, r + 4, r + 8,
 // ..., loads kQuadDepth of them at a time and forms each slot's term from a zero
 // accumulator (0 + term = term, and a sum that starts at +0 never holds -0, so adding the
@@ -1316,8 +1324,14 @@ __global__ __launch_bounds__(B) void k_list_gather(P pol, Sph s0, Scenes sc, uin
 // slots in slot order through DPP broadcasts, so each lane holds the particle's sums exactly
 // as the one-lane gather forms them; lane 0 finishes the particle. A small scene's sweep then
 // runs four times as many waves, each with a quarter of the pair arithmetic in its chain.
-// Policies whose pair() only adds a term to each accumulator component can take it.
+// Policies whose pair() only adds a term to each accumulator component (kAdditive) can take
+// it; the warm-start feature gathers, which count their pairs, cannot.
 constexpr int kQuadDepth = 8;   // slots per lane loaded at once: 32 per particle
+
+template <class P, class = void>
+struct Additive : std::false_type {};
+template <class P>
+struct Additive<P, std::void_t<decltype(P::kAdditive)>> : std::bool_constant<P::kAdditive && !P::kWalls> {};
 
 // x of quad lane u, in every lane of the quad
 __device__ __forceinline__ float quad_bcast(float x, int u) {
@@ -1406,7 +1420,7 @@ __global__ __launch_bounds__(B) void k_quad_gather(P pol, Sph s0, uint32_t n, co
                                                    const unsigned long long* __restrict__ prev,
                                                    unsigned long long thr,
                                                    unsigned long long* __restrict__ slot_out) {
-    static_assert(!P::kWalls, "wall samples take the one-lane gathers");
+    static_assert(Additive<P>::value, "only additive policies without wall samples");
     if (prev && *prev <= thr) return;
     const uint32_t wi = work_item<B>();
     const uint32_t t = wi >> 2, r = wi & 3u;
@@ -1575,11 +1589,30 @@ void launch_gather_b(fe_ctx* c, uint32_t block, const P& pol, const unsigned lon
     }
 }
 
+template <class P, int B>
+void launch_quad(fe_ctx* c, const P& pol, const unsigned long long* prev, unsigned long long thr,
+                 unsigned long long* slot_out) {
+    const uint64_t items = 4ull * c->n;
+    hipLaunchKernelGGL((k_quad_gather<P, B>), dim3((unsigned)((items + B - 1) / B)), dim3(B), 0, c->stream, pol,
+                       sph_of(c), c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr, c->nbr_count,
+                       c->list_valid ? 1u : 0u, prev, thr, slot_out);
+}
+
 template <class P>
 hipError_t run_list(fe_ctx* c, const char* name, int tid, const P& pol, const unsigned long long* prev = nullptr, unsigned long long thr = 0, uint32_t kind = 0,
                     unsigned long long* slot_out = nullptr) {
     Launch L(c, name);
     const uint32_t block = tid >= 0 ? c->tune[tid].block : 256u, items = tid >= 0 ? c->tune[tid].items : 1u;
+    if constexpr (Additive<P>::value) {
+        if (items == 8 && c->nscenes == 1) {   // four lanes per particle
+            switch (block) {
+                case 64: launch_quad<P, 64>(c, pol, prev, thr, slot_out); break;
+                case 128: launch_quad<P, 128>(c, pol, prev, thr, slot_out); break;
+                default: launch_quad<P, 256>(c, pol, prev, thr, slot_out); break;
+            }
+            return hipGetLastError();
+        }
+    }
     if (c->nscenes > 1)
         launch_gather_b<P, true>(c, block, pol, prev, thr, kind, slot_out);
     else
@@ -1609,27 +1642,9 @@ void launch_win_b(fe_ctx* c, uint32_t block, const P& pol, const unsigned long l
 
 // A Jacobi sweep: staged windows over the current list (tuned items 2), or per-particle
 // gathers of the listed pairs (items 1; the bucket walk when there is no current list).
-template <class P, int B>
-void launch_quad(fe_ctx* c, const P& pol, const unsigned long long* prev, unsigned long long thr,
-                 unsigned long long* slot_out) {
-    const uint64_t items = 4ull * c->n;
-    hipLaunchKernelGGL((k_quad_gather<P, B>), dim3((unsigned)((items + B - 1) / B)), dim3(B), 0, c->stream, pol,
-                       sph_of(c), c->n, c->range, c->pos_b, c->pos_m, c->sched[0], c->inv, c->nbr, c->nbr_count,
-                       c->list_valid ? 1u : 0u, prev, thr, slot_out);
-}
-
 template <class P>
 hipError_t run_sweep(fe_ctx* c, const char* name, int tid, const P& pol, const unsigned long long* prev, unsigned long long thr, uint32_t kind,
                      unsigned long long* slot_out) {
-    if (c->tune[tid].items == 8 && c->nscenes == 1) {   // four lanes per particle
-        Launch L(c, name);
-        switch (c->tune[tid].block) {
-            case 64: launch_quad<P, 64>(c, pol, prev, thr, slot_out); break;
-            case 128: launch_quad<P, 128>(c, pol, prev, thr, slot_out); break;
-            default: launch_quad<P, 256>(c, pol, prev, thr, slot_out); break;
-        }
-        return hipGetLastError();
-    }
     if (!c->list_valid || c->tune[tid].items != 2) return run_list(c, name, tid, pol, prev, thr, kind, slot_out);
     if (c->win_gen != c->list_gen) {   // the current list has no window tables yet
         Launch W(c, "window_tables");

@@ -781,13 +781,13 @@ bool tune_valid(int id, uint32_t block, uint32_t items) {
     if (id == kTunePressureIntegration || id == kTuneDivergenceIntegration)
         return pow2 && (items == 1 || items == 2 || items == 4);
     // the list gathers: items 1 gathers each listed pair from the mirrors with 4 loads in
-    // flight, items 4 with 16 (one scene, blocks of 64 to 256); for the sweeps items 2 stages
-    // one window per wave in LDS (1, 2 or 4 windows per block)
+    // flight, items 4 with 16, items 8 with four lanes per particle (one scene, blocks of 64 to
+    // 256); for the sweeps items 2 stages one window per wave in LDS (1, 2 or 4 windows per
+    // block)
     const bool small = block == 64 || block == 128 || block == 256;
-    // (and items 8 runs four lanes per particle, one scene)
     if (id == kTunePressureForce || id == kTunePressureUpdate)
         return (pow2 && items == 1) || (small && (items == 2 || items == 4 || items == 8));
-    return id >= 0 && id < kTuneCount && ((pow2 && items == 1) || (small && items == 4));
+    return id >= 0 && id < kTuneCount && ((pow2 && items == 1) || (small && (items == 4 || items == 8)));
 }
 
 hipError_t launch_unpack_start(fe_ctx* c, const uint32_t* in) {

@@ -338,8 +338,9 @@ fe_status fe_get_wall(const fe_ctx* ctx, fe_wall* out, int* attached);
  * with 16 in flight (block 64, 128 or 256; single-scene contexts without wall samples, a
  * batch or the wall mode runs variant 1), which shortens a small scene's chain of dependent
  * loads; for the two Jacobi sweeps, 2 stages each 64-particle window's neighbour union in LDS
- * and walks 16-bit lists (block 64, 128 or 256 = 1, 2 or 4 windows per block), and 8 runs four
- * lanes per particle (block 64, 128 or 256; single-scene contexts, otherwise variant 1), each
+ * and walks 16-bit lists (block 64, 128 or 256 = 1, 2 or 4 windows per block). For every list
+ * gather, 8 runs four lanes per particle (block 64, 128 or 256; single-scene contexts without
+ * wall samples, otherwise variant 1; the warm-start feature passes run variant 1), each lane
  * forming the terms of every fourth listed pair, added in list order across the four lanes.
  * fe_autotune tunes variants 1, 4 and 8 (variant 2 builds window tables once per list, which
  * timing sweep launches alone would not count; fe_tune_set selects it). In converge mode it

@@ -1,8 +1,8 @@
 """The small-scene launch variants on the GPU, through the C ABI:
 - deep list gathers (tune items 4): 16 listed neighbours' records in flight per work-item
   instead of 4 (one scene, blocks of 64 to 256);
-- four lanes per particle in the two Jacobi sweeps (items 8): each lane forms the terms of
-  every fourth listed slot and the quad adds them in slot order;
+- four lanes per particle (items 8) in the two Jacobi sweeps and the other list gathers: each
+  lane forms the terms of every fourth listed slot and the quad adds them in slot order;
 - build_list over windows of 8 schedule positions per wave (items 8) instead of 64.
 Each visits every particle's pairs in the same list order and builds the same lists, so
 fields, grid tables, stats and sweep counts equal those of the default launches bit for bit,
@@ -49,9 +49,11 @@ def deep(sim, block):
         sim.set_tune(k, block, 4)
 
 
-def small(sim, block):
-    """Every small-scene variant: deep gathers, quad sweeps, 8-position list windows."""
-    deep(sim, block)
+def small(sim, block, gathers=4):
+    """Every small-scene variant: deep (gathers=4) or four-lane (gathers=8) list gathers, quad
+    sweeps, 8-position list windows."""
+    for k in GATHERS:
+        sim.set_tune(k, block, gathers)
     sim.set_tune("pressure_force", block, 8)
     sim.set_tune("pressure_update", block, 8)
     sim.set_tune("build_list", block, 8)
@@ -260,3 +262,72 @@ def test_batch_small_windows_and_default_sweeps():
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("mode", [fe.FE_SOLVER_FIXED, fe.FE_SOLVER_CONVERGE])
+@pytest.mark.parametrize("block", [64, 256])
+def test_four_lane_gathers_match_default_32k(mode, block):
+    w = scene.dam_break_32k(seed=12)
+    with fe.Simulation(w.params, w.positions, device=0, options=opts(mode)) as a, \
+            fe.Simulation(w.params, w.positions, device=0, options=opts(mode)) as b:
+        small(b, block, gathers=8)
+        assert all(e["items"] == 8 for k, e in b.tune_entries().items() if k in GATHERS)
+        a.run(5)
+        b.run(5)
+        compare(a, b, f"mode {mode} block {block}")
+
+
+def test_four_lane_gathers_match_oracle_with_overflow(oracle_mod):
+    """Every list gather four lanes per particle, stage by stage against the oracle, with the
+    clump whose lists overflow (walked by lane 0 alone), then two whole frames."""
+    rng = np.random.default_rng(13)
+    w = scene.dam_break_32k(seed=4)
+    pos = w.positions.copy()
+    pos[:300] = np.array([0.5, 0.3, 0.5], np.float32) + rng.uniform(0, 0.02, (300, 3)).astype(np.float32)
+    o = opts(fe.FE_SOLVER_FIXED)
+    with fe.Simulation(w.params, pos, device=0, options=o) as s:
+        small(s, 128, gathers=8)
+        orc = oracle_mod.Oracle(w.params, pos, options=o)
+        try:
+            for x in (s, orc):
+                x.neighbor_search()
+                x.density_alpha()
+                x.viscosity()
+                x.density_source()
+                for _ in range(2):
+                    x.pressure_force()
+                    x.pressure_update()
+                x.divergence_source()
+            for name in ("densities", "factors", "velocity_a", "source_terms", "pressure_accelerations", "pressures"):
+                same("clump " + name, s.download(name), orc.download(name))
+            for f in range(2):
+                s.step()
+                orc.step()
+                for name in FIELDS:
+                    same(f"frame {f} {name} vs oracle", s.download(name), orc.download(name))
+        finally:
+            orc.close()
+
+
+def test_four_lane_gathers_app_scene_graph_frames():
+    w = scene.app_scene()
+    o = opts(fe.FE_SOLVER_FIXED)
+    sims = []
+    try:
+        for _ in range(2):
+            x = fe.Simulation(w.params, w.positions, device=0, options=o)
+            x.set_wall(fe.CollisionBox.from_params(w.params, *w.wall_wave))
+            sims.append(x)
+        a, b = sims
+        small(b, 64, gathers=8)
+        b.use_graph(True)
+        for f in range(4):
+            a.step()
+            b.step()
+            compare(a, b, f"app frame {f}")
+        a.run(20)
+        b.run(20)
+        compare(a, b, "app after 24 frames")
+    finally:
+        for x in sims:
+            x.close()
