@@ -461,9 +461,14 @@ fe_status neighbor_search(fe_ctx* c) {
     c->ends_dirty = false;
     c->list_valid = false;
     c->ws.feat_fresh = false;
-    // gather schedule: particles in Morton order of their cells (keys from offsets/reorder)
-    FE_HIP(fe::radix_sort_pairs(c, c->mkeys, c->sched, c->n, c->morton_bits + c->scene_bits, false),
-           "schedule sort");
+    // gather schedule: particles in Morton order of their cells (keys from offsets/reorder).
+    // A small single scene keeps hash order (offsets/reorder wrote the identity): its fields
+    // sit in the L2 caches whatever the order, and a cell's particles are consecutive in hash
+    // order as well. (With the learned warm start the Morton order stays, so that its samples
+    // are drawn as before.)
+    if (!(c->nscenes == 1 && c->n <= fe::kHashScheduleMax && !c->ws.alloc))
+        FE_HIP(fe::radix_sort_pairs(c, c->mkeys, c->sched, c->n, c->morton_bits + c->scene_bits, false),
+               "schedule sort");
     FE_HIP(fe::launch_schedule_mirrors(c), "schedule mirrors");
     c->ns_fresh = true;
     c->perm_valid = true;

@@ -17,6 +17,7 @@ constexpr uint32_t kBlock = 256;          // threads per block of the per-partic
 constexpr uint32_t kMaxSolveIters = 1024; // residual slots per solve
 constexpr uint32_t kStatsBlocks = 1024;   // blocks of the stats reduction
 constexpr uint32_t kMaxNbr = 64;          // neighbour-list slots per particle
+constexpr uint32_t kHashScheduleMax = 1u << 16;   // single scenes up to this size gather in hash order
 constexpr uint32_t kTabRing = 4;          // pinned per-frame scene tables of a batch with moving walls
 // schedule-ordered mirrors of the fields the gathers read from neighbours, and of the
 // per-particle solver fields the Jacobi sweeps read for their own particle
