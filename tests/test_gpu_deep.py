@@ -331,3 +331,30 @@ def test_four_lane_gathers_app_scene_graph_frames():
     finally:
         for x in sims:
             x.close()
+
+
+def test_four_lane_gathers_with_a_stale_list(oracle_mod):
+    """position_b uploaded after the list was built: the list is stale, so every four-lane
+    gather walks the buckets in lane 0, with the moved positions, as the oracle does."""
+    rng = np.random.default_rng(14)
+    w = scene.dam_break_32k(seed=5)
+    o = opts(fe.FE_SOLVER_FIXED)
+    with fe.Simulation(w.params, w.positions, device=0, options=o) as s:
+        small(s, 64, gathers=8)
+        orc = oracle_mod.Oracle(w.params, w.positions, options=o)
+        try:
+            for x in (s, orc):
+                x.neighbor_search()
+                x.density_alpha()
+            pb = orc.download("position_b")
+            pb[:, :3] += rng.normal(0, 0.002, (pb.shape[0], 3)).astype(np.float32)
+            for x in (s, orc):
+                x.upload("position_b", pb)
+                x.viscosity()
+                x.density_source()
+                x.pressure_force()
+                x.pressure_update()
+            for name in ("velocity_a", "source_terms", "pressure_accelerations", "pressures"):
+                same("stale-list " + name, s.download(name), orc.download(name))
+        finally:
+            orc.close()
