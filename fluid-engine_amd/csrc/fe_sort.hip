@@ -34,7 +34,9 @@ namespace {
 constexpr int kSortThreads = 256;
 constexpr int kWaves = kSortThreads / 64;
 constexpr int kHistItems = 16;
-constexpr int kMinTile = 4096;                 // the smallest scatter tile (scratch sizing)
+constexpr int kMinTile = 4096;                 // the smallest scatter tile above kSmallSort keys
+constexpr uint32_t kSmallSort = 1u << 16;      // sorts up to this size scatter in 1024-key tiles
+constexpr uint32_t kSmallTile = 1024;
 constexpr uint32_t kGroup = 32;                // tiles per look-back group
 constexpr uint32_t kGroupBatch = 8;            // group words read per poll
 static_assert(kGroup == kSortGroupTiles, "group size");
@@ -437,7 +439,10 @@ __global__ __launch_bounds__(kBitonicThreads) void k_bitonic_local(uint32_t* __r
 }  // namespace
 
 uint32_t sort_max_tiles(uint32_t max_n) {
-    const uint32_t t = (uint32_t)(((uint64_t)max_n + kMinTile - 1) / kMinTile);
+    const uint32_t small = max_n < kSmallSort ? max_n : kSmallSort;
+    const uint32_t ts = (small + kSmallTile - 1) / kSmallTile;
+    uint32_t t = (uint32_t)(((uint64_t)max_n + kMinTile - 1) / kMinTile);
+    t = t > ts ? t : ts;
     return t ? t : 1u;
 }
 
@@ -492,7 +497,10 @@ namespace {
 struct ScatterCfg {
     uint32_t threads, items;
 };
-ScatterCfg scatter_cfg(uint32_t n) { return n <= (1u << 21) ? ScatterCfg{512, 8} : ScatterCfg{512, 16}; }
+ScatterCfg scatter_cfg(uint32_t n) {
+    if (n <= kSmallSort) return ScatterCfg{256, 4};   // more, shorter tiles for a small sort
+    return n <= (1u << 21) ? ScatterCfg{512, 8} : ScatterCfg{512, 16};
+}
 // 256 threads x 16 keys per chunk: a 2^20-key sort counts in 256 blocks, one per CU
 constexpr uint32_t kHistThreads = 256;
 constexpr uint32_t kHistBlocks = 512;
@@ -530,7 +538,8 @@ hipError_t radix_sort_core(const SortScratch& s, hipStream_t stream, uint32_t* k
 #define FE_SCATTER(T, I)                                                                                      \
     hipLaunchKernelGGL((k_onesweep_scatter<T, I>), dim3(tiles), dim3(T), 0, stream, keys[0], vals[0], keys[1], \
                        vals[1], n, p, passes, tiles, s.hist, tile_words, group_words, s.ctrl)
-            if (cfg.items == 8) FE_SCATTER(512, 8);
+            if (cfg.items == 4) FE_SCATTER(256, 4);
+            else if (cfg.items == 8) FE_SCATTER(512, 8);
             else FE_SCATTER(512, 16);
 #undef FE_SCATTER
         }
