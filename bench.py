@@ -91,6 +91,9 @@ def parse():
     ap.add_argument("--learned-eval", type=int, default=400, help="frames compared after training")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--app-eager-only", action="store_true",
+                    help="only time the reference app's 25,000-particle frame, eagerly launched, physics only "
+                         "(prints that section's JSON; for profiling its kernels alone)")
     return ap.parse_args()
 
 
@@ -141,6 +144,11 @@ def main():
     args = parse()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.app_eager_only:
+        import fluid_engine as fe
+        from fluid_engine import scene
+        print(json.dumps(app_frame_measurement(fe, scene, False, contexts=((False, False),))), flush=True)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -403,12 +411,13 @@ def divergence_order_measurement(fe, w, opts, steps, warmup, tuned=None):
     return out
 
 
-def app_frame_measurement(fe, scene, with_cpu, warmup=20, frames=200):
+def app_frame_measurement(fe, scene, with_cpu, warmup=20, frames=200,
+                          contexts=((False, True), (True, True), (False, False))):
     """The reference app's own frame (src/core/scene.rs:14-86, src/renderer/mod.rs:227-391): its
     25,000-particle scene, the min.x wall oscillating 0.3 m at 0.5 Hz, fixed 4/4 sweeps, radix
     sort, dt = max_dt = 5 ms, and the raymarching density splat (128^3) after each frame.
     Frames/s over `frames` frames (HIP events), with and without the splat, each context
-    autotuned after its warm-up."""
+    autotuned after its warm-up; `contexts` lists the (splat, graph) pairs to time."""
     w = scene.app_scene()
     o = fe.default_options()
     o.sort_algorithm = fe.FE_SORT_RADIX
@@ -416,7 +425,7 @@ def app_frame_measurement(fe, scene, with_cpu, warmup=20, frames=200):
            "frames": f"{warmup} warm-up, autotune, 5 frames + {frames} timed"}
     # the reference records each frame into one command buffer; here the frame is replayed
     # from a captured HIP graph (fe_use_graph), and also timed eagerly
-    for splat, graph in ((False, True), (True, True), (False, False)):
+    for splat, graph in contexts:
         sim = fe.Simulation(w.params, w.positions, device=0, options=o)
         sim.set_wall(fe.CollisionBox.from_params(w.params, *w.wall_wave))
         for _ in range(warmup):
