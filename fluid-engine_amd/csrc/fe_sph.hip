@@ -403,40 +403,82 @@ __global__ __launch_bounds__(kBlock) void k_stats_final(const uint32_t* __restri
 }
 
 // stats.comp:18-30 for a batch: each scene's three values by atomics (max and wrapping u32
-// sums are order-independent, as in the reference), one set per wave unless the wave
-// straddles two scenes. The per-scene triples are zeroed first (k_fill_u32).
+// sums are order-independent, as in the reference). A scene's particles are contiguous in
+// index, so a thread folds its kStatsItems particles (strided by the block, for coalesced
+// loads) into running values, adding them to the scene's words only where the scene changes;
+// a wave whose lanes end in one scene adds once, and a block whose waves all do so adds
+// once. That keeps the atomics on the few words of each scene to about three per block
+// (one set per wave put 175,000 atomics on 192 words for 3.7M particles and took 0.83 ms).
+// The per-scene triples are zeroed first (k_fill_u32).
+constexpr uint32_t kStatsItems = 16;
+
+__device__ __forceinline__ void add_scene_stats(uint32_t* __restrict__ stats, uint32_t scene, uint32_t mx,
+                                                uint32_t s1, uint32_t s2) {
+    atomicMax(stats + 3 * scene, mx);
+    atomicAdd(stats + 3 * scene + 1, s1);
+    atomicAdd(stats + 3 * scene + 2, s2);
+}
+
 __global__ __launch_bounds__(kBlock) void k_stats_scenes(uint32_t n, Scenes sc, const float4* __restrict__ vel,
                                                          const float* __restrict__ dens,
                                                          const float* __restrict__ src,
                                                          uint32_t* __restrict__ stats) {
-    const uint32_t i = gid();
-    uint32_t mx = 0, s1 = 0, s2 = 0, scene = 0;
-    if (i < n) {
-        scene = sc.sid[i];
-        const float rho0 = sc.tab[scene].rho0;
+    constexpr uint32_t kWaves = kBlock / 64, kMixed = kSentinel - 1u;
+    __shared__ uint32_t wsc[kWaves], wmx[kWaves], ws1[kWaves], ws2[kWaves];
+    const uint32_t base = blockIdx.x * (kBlock * kStatsItems) + threadIdx.x;
+    uint32_t mx = 0, s1 = 0, s2 = 0, cur = kSentinel;   // kSentinel: no particle yet
+    float rho0 = 0.0f;
+    for (uint32_t k = 0; k < kStatsItems; k++) {
+        const uint32_t i = base + k * kBlock;
+        if (i >= n) break;
+        const uint32_t scene = sc.sid[i];
+        if (scene != cur) {
+            if (cur != kSentinel) add_scene_stats(stats, cur, mx, s1, s2);
+            cur = scene;
+            rho0 = sc.tab[scene].rho0;
+            mx = s1 = s2 = 0;
+        }
         const float4 v = vel[i];
         const float speed = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
-        mx = __float_as_uint(speed);
-        s1 = min(to_u32_sat(fabsf(dens[i] - rho0) * 1.0f), 10000u);
-        s2 = min(to_u32_sat(fabsf(src[i]) * 10.0f), 10000u);
+        mx = max(mx, __float_as_uint(speed));
+        s1 += min(to_u32_sat(fabsf(dens[i] - rho0) * 1.0f), 10000u);
+        s2 += min(to_u32_sat(fabsf(src[i]) * 10.0f), 10000u);
     }
-    const uint32_t first = __shfl(scene, 0, 64);
-    if (__all(scene == first)) {
+    const uint32_t first = __shfl(cur, 0, 64);
+    const bool uni = __all(cur == first);   // the wave's lanes end in one scene (or none)
+    if (uni) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
             s1 += (uint32_t)__shfl_xor((int)s1, off, 64);
             s2 += (uint32_t)__shfl_xor((int)s2, off, 64);
         }
-        if ((threadIdx.x & 63) == 0) {
-            atomicMax(stats + 3 * first, mx);
-            atomicAdd(stats + 3 * first + 1, s1);
-            atomicAdd(stats + 3 * first + 2, s2);
+    }
+    const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
+    if (l == 0) {
+        wsc[w] = uni ? first : kMixed;
+        wmx[w] = mx;
+        ws1[w] = s1;
+        ws2[w] = s2;
+    }
+    __syncthreads();
+    bool block_uni = wsc[0] < kMixed;
+#pragma unroll
+    for (uint32_t u = 1; u < kWaves; u++) block_uni = block_uni && wsc[u] == wsc[0];
+    if (block_uni) {
+        if (threadIdx.x == 0) {
+            uint32_t bm = wmx[0], b1 = ws1[0], b2 = ws2[0];
+            for (uint32_t u = 1; u < kWaves; u++) {
+                bm = max(bm, wmx[u]);
+                b1 += ws1[u];
+                b2 += ws2[u];
+            }
+            add_scene_stats(stats, wsc[0], bm, b1, b2);
         }
-    } else if (i < n) {
-        atomicMax(stats + 3 * scene, mx);
-        atomicAdd(stats + 3 * scene + 1, s1);
-        atomicAdd(stats + 3 * scene + 2, s2);
+    } else if (uni) {
+        if (l == 0 && first != kSentinel) add_scene_stats(stats, first, mx, s1, s2);
+    } else if (cur != kSentinel) {
+        add_scene_stats(stats, cur, mx, s1, s2);
     }
 }
 
@@ -717,8 +759,9 @@ hipError_t launch_stats(fe_ctx* c) {
         hipLaunchKernelGGL(k_fill_u32, grid_for(3 * c->nscenes), dim3(kBlock), 0, c->stream, c->scene_stats,
                            3 * c->nscenes);
         Launch L(c, "stats");
-        hipLaunchKernelGGL(k_stats_scenes, grid_for(c->n), dim3(kBlock), 0, c->stream, c->n, scenes_of(c), c->vel_a,
-                           c->dens, c->src, c->scene_stats);
+        const uint32_t per = kBlock * kStatsItems;
+        hipLaunchKernelGGL(k_stats_scenes, dim3((c->n + per - 1) / per), dim3(kBlock), 0, c->stream, c->n,
+                           scenes_of(c), c->vel_a, c->dens, c->src, c->scene_stats);
         return hipGetLastError();
     }
     uint32_t blocks = (c->n + kBlock - 1) / kBlock;
