@@ -1204,8 +1204,9 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v
 }
 
 // A sweep block's residual output, summed into the sweep's slot by integer atomics (so the
-// order does not matter): one per block for one scene; for a batch one per wave into its
-// scene's slot, unless the wave straddles two scenes.
+// order does not matter): one per block for one scene; for a batch one per block into its
+// scene's slot when all its waves lie in that scene, else one per wave, or per lane where a
+// wave straddles two scenes.
 template <bool kBatch, int B>
 __device__ __forceinline__ void sweep_epilogue(unsigned long long q, uint32_t scene,
                                                unsigned long long* __restrict__ slot_out) {
@@ -1215,10 +1216,34 @@ __device__ __forceinline__ void sweep_epilogue(unsigned long long q, uint32_t sc
         if (threadIdx.x == 0 && b) atomicAdd(slot_out, b);
     } else {
         const uint32_t s_first = __shfl(scene, 0, 64);
-        if (__all(scene == s_first)) {
-            unsigned long long v = q;
+        const bool uni = __all(scene == s_first);
+        unsigned long long v = q;
+        if (uni) {
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        }
+        if constexpr (B > 64) {
+            __shared__ uint32_t wsc[B / 64];
+            __shared__ unsigned long long wv[B / 64];
+            const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+            if (lane == 0) {
+                wsc[w] = uni ? s_first : kSentinel;
+                wv[w] = v;
+            }
+            __syncthreads();
+            bool block_uni = wsc[0] != kSentinel;
+#pragma unroll
+            for (int u = 1; u < B / 64; u++) block_uni = block_uni && wsc[u] == wsc[0];
+            if (block_uni) {
+                if (threadIdx.x == 0) {
+                    unsigned long long t = 0;
+                    for (int u = 0; u < B / 64; u++) t += wv[u];
+                    if (t) atomicAdd(slot_out + (size_t)wsc[0] * kMaxSolveIters, t);
+                }
+                return;
+            }
+        }
+        if (uni) {
             if ((threadIdx.x & 63) == 0 && v) atomicAdd(slot_out + (size_t)s_first * kMaxSolveIters, v);
         } else if (q) {
             atomicAdd(slot_out + (size_t)scene * kMaxSolveIters, q);
