@@ -1092,21 +1092,30 @@ struct ResidualReport : PressureUpdate {
 };
 
 // Sums of a report: |r|, max(0, r) and max(0, r) where the final pressure is > 0, each as
-// floor(x dt / rho0 2^24), plus the largest |r| term (slots 0..3).
+// floor(x dt / rho0 2^24), plus the largest |r| term (slots 0..3). Each thread folds
+// kReportItems particles (strided by the block), and the block adds once: integer sums and a
+// maximum, so the order does not change them, and four atomics per block instead of per wave
+// keep the four words from serialising the launch.
+constexpr uint32_t kReportItems = 16;
+
 __global__ __launch_bounds__(kBlock) void k_residual_reduce(const float* __restrict__ err, const float* __restrict__ pres_m,
                                                             uint32_t n, float dt, float rho0,
                                                             unsigned long long* __restrict__ out) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    unsigned long long a = 0, e = 0, ea = 0;
-    if (t < n) {
+    __shared__ unsigned long long part[4][kBlock / 64];
+    const uint32_t base = blockIdx.x * (kBlock * kReportItems) + threadIdx.x;
+    const float scale = dt / rho0 * 16777216.0f;
+    unsigned long long a = 0, e = 0, ea = 0, mx = 0;
+    for (uint32_t k = 0; k < kReportItems; k++) {
+        const uint32_t t = base + k * kBlock;
+        if (t >= n) break;
         const float r = err[t];
-        const float scale = dt / rho0 * 16777216.0f;
         const float ab = fminf(fabsf(r) * scale, 1.0e15f), ex = fminf(fmaxf(r, 0.0f) * scale, 1.0e15f);
-        a = (unsigned long long)ab;
-        e = (unsigned long long)ex;
-        ea = pres_m[t] > 0.0f ? e : 0ull;
+        const unsigned long long at = (unsigned long long)ab, et = (unsigned long long)ex;
+        a += at;
+        e += et;
+        ea += pres_m[t] > 0.0f ? et : 0ull;
+        mx = at > mx ? at : mx;
     }
-    unsigned long long mx = a;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_xor(a, off, 64);
@@ -1115,7 +1124,21 @@ __global__ __launch_bounds__(kBlock) void k_residual_reduce(const float* __restr
         const unsigned long long o = __shfl_xor(mx, off, 64);
         mx = o > mx ? o : mx;
     }
+    const uint32_t w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        part[0][w] = a;
+        part[1][w] = e;
+        part[2][w] = ea;
+        part[3][w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t u = 1; u < kBlock / 64; u++) {
+            a += part[0][u];
+            e += part[1][u];
+            ea += part[2][u];
+            mx = part[3][u] > mx ? part[3][u] : mx;
+        }
         if (a) atomicAdd(out, a);
         if (e) atomicAdd(out + 1, e);
         if (ea) atomicAdd(out + 2, ea);
@@ -1994,8 +2017,9 @@ hipError_t launch_residual_report(fe_ctx* c, int kind) {
     e = walls_on(c) ? run_walled(c, "residual_report", -1, pol) : run_list(c, "residual_report", kTunePressureUpdate, pol);
     if (e != hipSuccess) return e;
     Launch L(c, "residual_reduce");
-    hipLaunchKernelGGL(k_residual_reduce, grid_for(c->n), dim3(kBlock), 0, c->stream, c->report_err, c->pres_m, c->n,
-                       c->params.dt, c->params.target_density, c->report + 4 * kind);
+    const uint32_t per = kBlock * kReportItems;
+    hipLaunchKernelGGL(k_residual_reduce, dim3((c->n + per - 1) / per), dim3(kBlock), 0, c->stream, c->report_err,
+                       c->pres_m, c->n, c->params.dt, c->params.target_density, c->report + 4 * kind);
     return hipGetLastError();
 }
 
